@@ -1,0 +1,262 @@
+"""ORACLE — test infrastructure only.
+
+CPU restatement of the reference's AC-CAS path (pernalk/GAMMCOR, in-core
+variant), used by tests/, by ``__graft_entry__.smoke()`` and by bench.py's
+cpu_baseline leg as the checker.  The product path (gammcor_amd) never imports
+it.  Each function cites the reference lines it follows.
+
+Pinning (what independently fixes each function; see tests/test_oracle.py):
+  gauleg          numpy.polynomial.legendre.leggauss; compiled reference
+  select_pairs    compiled reference (ACCAS end to end) and the NDimX closed form
+  two_no1         closed form from the Cholesky factors; brute-force einsum at
+                  small N; compiled reference TwoNO1 checksums
+  ab_cas          compiled reference AB_CAS (full matrices at small N);
+                  alpha-affinity identity
+  erpasymm1       scipy eigh; omega^2 = eig(A+ A-); the matrix-function
+                  identity sum_K Y_K Y_K^T = 1/2 S (S A- S)^(-1/2) S
+  aceneerpa       compiled reference ACEInteg W(alpha) values
+  acecorr         compiled reference ECorr (8-point grid and ACCAS 5-point)
+
+Deliberate difference: TwoNO1 screens |value| <= 1e-12 terms (misc.f:175,
+212, 247, 281); the restatement does not.  The seeded integrals are O(1e-2),
+so the screen changes no value by more than rounding.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SMALL_E = 1.0e-3      # acfd.f:4976
+BIG_E = 1.0e8
+SMALL_OM2 = 1.0e-6    # interpa.f:1663
+
+
+# ---------------------------------------------------------------- GauLeg
+
+def gauleg(x1, x2, n):
+    """Gauss-Legendre nodes/weights on [x1, x2] by Newton iteration on P_n
+    (misc.f:828-865, Numerical Recipes 4.5)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    m = (n + 1) // 2
+    xm = 0.5 * (x2 + x1)
+    xl = 0.5 * (x2 - x1)
+    for i in range(1, m + 1):
+        z = np.cos(3.141592654 * (i - 0.25) / (n + 0.5))
+        while True:
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p3 = p2
+                p2 = p1
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z1 = z
+            z = z1 - p1 / pp
+            if abs(z - z1) <= 3.0e-14:
+                break
+        x[i - 1] = xm - xl * z
+        x[n - i] = xm + xl * z
+        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
+        w[n - i] = w[i - 1]
+    return x, w
+
+
+# ---------------------------------------------------------------- pairs
+
+def select_pairs(occ, nele, thr_sel_act=1e-8, thr_qvirt=1e-7, iflcore=1):
+    """ERPA pair list IndN (1-based, p > q) of accas.f:30-99."""
+    nb = len(occ)
+    indaux = np.where(np.arange(nb) < nele, 0, 2)
+    for i in range(nb):
+        if occ[i] < 1.0 and occ[i] != 0.0:
+            indaux[i] = 1
+    pairs = []
+    for i in range(nb):
+        for j in range(i):
+            s = indaux[i] + indaux[j]
+            if s == 0 or s == 4:
+                continue
+            if indaux[i] == 1 and indaux[j] == 1 and abs(occ[i] - occ[j]) / occ[i] < thr_sel_act:
+                continue
+            if iflcore == 1 or (occ[i] != 1.0 and occ[j] != 1.0):
+                if abs(occ[i] + occ[j] - 2.0) > 1e-10 and abs(occ[i] + occ[j]) > thr_qvirt:
+                    pairs.append((i + 1, j + 1))
+    return np.array(pairs, dtype=np.int32).reshape(-1, 2)
+
+
+# ---------------------------------------------------------------- transform
+
+def _pair(i, j):
+    a = np.maximum(i, j)
+    b = np.minimum(i, j)
+    return a * (a + 1) // 2 + b
+
+
+def naddr3(p, q, r, s):
+    """0-based NAddr3 (misc.f:505-525)."""
+    return _pair(_pair(p, q), _pair(r, s))
+
+
+def two_no1(ao_packed, u, chunk=1024):
+    """(ab|cd) = sum U[a,m]U[b,n]U[c,l]U[d,s] (mn|ls), 8-fold packed in and
+    out (misc.f:166-317: first/second index at 189-254, third/fourth at
+    263-311).  Done as two half-transforms of the T x T pair matrix."""
+    n = u.shape[0]
+    t = n * (n + 1) // 2
+    il, jl = np.tril_indices(n)
+    # full T x T pair matrix from the packed triangle
+    v = np.zeros((t, t))
+    ii, jj = np.tril_indices(t)
+    v[ii, jj] = ao_packed
+    v[jj, ii] = ao_packed
+
+    def half(m):
+        out = np.empty_like(m)
+        for c0 in range(0, t, chunk):
+            c1 = min(t, c0 + chunk)
+            blk = np.zeros((n, n, c1 - c0))
+            blk[il, jl, :] = m[:, c0:c1]
+            blk[jl, il, :] = m[:, c0:c1]
+            x = np.einsum("am,mnc->anc", u, blk, optimize=True)
+            y = np.einsum("anc,bn->abc", x, u, optimize=True)
+            out[:, c0:c1] = y[il, jl, :]
+        return out
+
+    h = half(v)          # rows ab (NO), cols kl (AO)
+    h = half(h.T)        # rows cd (NO), cols ab (NO)
+    return h[ii, jj]     # packed (ab|cd), ab >= cd  (h symmetric)
+
+
+# ---------------------------------------------------------------- AB_CAS
+
+_lib = None
+
+
+def _abcas_lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle_abcas.so")
+        if not os.path.exists(path):
+            import subprocess
+            subprocess.run(["make", "-C", _HERE, "liboracle_abcas.so"], check=True,
+                           stdout=subprocess.DEVNULL)
+        _lib = ctypes.CDLL(path)
+        d = ctypes.POINTER(ctypes.c_double)
+        i = ctypes.POINTER(ctypes.c_int)
+        _lib.oracle_ab_cas.argtypes = [ctypes.c_int] * 3 + [d, d, d, d, d, i, ctypes.c_int, i,
+                                                             ctypes.c_double, d, d, d, d]
+        _lib.oracle_ab_cas.restype = ctypes.c_int
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def ab_cas(nbasis, ninact, nact, occ, xone, twono, gamma_act, igem, indn, alpha, ure=None):
+    """A+B, A-B of the alpha-Hamiltonian (interpa.f:3659-4104).  Returns
+    (abplus, abmin, ecasscf, cicoef) with column-major (Fortran) matrices."""
+    lib = _abcas_lib()
+    n = len(indn)
+    if ure is None:
+        ure = np.eye(nbasis)
+    ure = np.asfortranarray(ure, dtype=np.float64)
+    occ = np.ascontiguousarray(occ, dtype=np.float64)
+    xone = np.ascontiguousarray(xone, dtype=np.float64)
+    twono = np.ascontiguousarray(twono, dtype=np.float64)
+    g = np.ascontiguousarray(gamma_act, dtype=np.float64)
+    ig = np.ascontiguousarray(igem, dtype=np.int32)
+    ind = np.ascontiguousarray(indn, dtype=np.int32)
+    ap = np.zeros((n, n), order="F")
+    am = np.zeros((n, n), order="F")
+    et = np.zeros(1)
+    ci = np.zeros(nbasis)
+    rc = lib.oracle_ab_cas(nbasis, ninact, nact, _dp(occ), _dp(xone), _dp(ure), _dp(twono), _dp(g),
+                           _ip(ig), n, _ip(ind), float(alpha), _dp(ap), _dp(am), _dp(et), _dp(ci))
+    if rc != 0:
+        raise MemoryError("oracle_ab_cas failed")
+    return ap, am, float(et[0]), ci
+
+
+# ---------------------------------------------------------------- ERPASYMM1
+
+def erpasymm1(abplus, abmin):
+    """Symmetric ERPA solve of interpa.f:1652-1781.  Returns (Y, eig) where
+    Y[:, K] are the normalised Y vectors and eig[K] = omega_K when
+    omega_K^2 > 1e-6, else omega_K^2 (ascending order)."""
+    ap = 0.5 * (abplus + abplus.T)            # 1680-1687
+    am = 0.5 * (abmin + abmin.T)
+    lam, v = np.linalg.eigh(ap)               # Diag8 -> DSYEVD (misc.f:541-616)
+    s = (v * np.sqrt(np.abs(lam))[None, :]) @ v.T    # 1695-1716, |lambda| semantics
+    m = s @ (am @ s)                          # 1725-1726
+    m = 0.5 * (m + m.T)
+    om2, z = np.linalg.eigh(m)                # 1729
+    y = s @ z                                 # 1734-1740
+    eig = om2.copy()
+    pos = om2 > SMALL_OM2                     # 1748-1776
+    eig[pos] = np.sqrt(om2[pos])
+    nrm = 2.0 / eig[pos] * np.einsum("ik,ij,jk->k", y[:, pos], am, y[:, pos])
+    scale = np.where(nrm > 0.0, 1.0 / np.sqrt(np.where(nrm > 0.0, nrm, 1.0)), 0.0)
+    y[:, pos] *= scale[None, :]
+    return y, eig, int(np.sum(lam < 0.0))
+
+
+# ---------------------------------------------------------------- ACEneERPA
+
+def pair_integrals(twono, indn):
+    """G[I,J] = (p r|q s) for I=(p,r), J=(q,s) = IndN columns (acfd.f:5030)."""
+    p = indn[:, 0] - 1
+    r = indn[:, 1] - 1
+    return twono[naddr3(p[:, None], r[:, None], p[None, :], r[None, :])]
+
+
+def aceneerpa(y, eig, twono, occ, cicoef, igem, indn, g=None):
+    """W(alpha) of acfd.f:4968-5043 (returned without the caller's weight)."""
+    p = indn[:, 0] - 1
+    r = indn[:, 1] - 1
+    keep = (eig > SMALL_E) & (eig < BIG_E)
+    yk = y[:, keep]
+    pmat = yk @ yk.T                                           # SumY
+    c = cicoef
+    cij = c[p] + c[r]
+    aux = 2.0 * cij[:, None] * cij[None, :] * pmat
+    aux[np.diag_indices_from(aux)] -= occ[p] * (1.0 - occ[r]) + occ[r] * (1.0 - occ[p])
+    same = (igem[p] == igem[r])[:, None] & (igem[p][:, None] == igem[p][None, :]) & \
+        (igem[p] == igem[r])[None, :]
+    if g is None:
+        g = pair_integrals(twono, indn)
+    return float(np.sum(np.where(same, 0.0, aux * g))), int(np.sum(~keep))
+
+
+# ---------------------------------------------------------------- drivers
+
+def aceinteg(sysd, twono, alpha, indn, g=None):
+    """One alpha point (acfd.f:4832-4966): AB_CAS, ERPASYMM1, ACEneERPA."""
+    ap, am, ecas, ci = ab_cas(sysd["nbasis"], sysd["ninact"], sysd["nact"], sysd["occ"], sysd["xone"],
+                              twono, sysd["gamma_act"], sysd["igem"], indn, alpha)
+    y, eig, nneg = erpasymm1(ap, am)
+    w, nskip = aceneerpa(y, eig, twono, sysd["occ"], ci, sysd["igem"], indn, g)
+    return dict(w=w, eig=eig, ecasscf=ecas, nneg_aplus=nneg, nskip=nskip)
+
+
+def acecorr(sysd, twono, alphas, weights):
+    """ECorr = sum_i w_i W(alpha_i) (acfd.f:161-201)."""
+    indn = select_pairs(sysd["occ"], sysd["nele"])
+    g = pair_integrals(twono, indn)
+    res = [aceinteg(sysd, twono, a, indn, g) for a in alphas]
+    ecorr = float(sum(wi * r["w"] for wi, r in zip(weights, res)))
+    return dict(ecorr=ecorr, walpha=[r["w"] for r in res], ecasscf=res[0]["ecasscf"],
+                eig=[r["eig"] for r in res], ndimx=len(indn),
+                nneg_aplus=[r["nneg_aplus"] for r in res])
+
+
+def system_dict(sysm):
+    return dict(nbasis=sysm.nbasis, ninact=sysm.ninact, nact=sysm.nact, occ=sysm.occ, xone=sysm.xone,
+                gamma_act=sysm.gamma_act, igem=sysm.igem, nele=sysm.nele)

@@ -1,0 +1,194 @@
+C     Driver for the compiled reference (oracle/_ref).  Test
+C     infrastructure only: it fills the reference's COMMON blocks from a
+C     seeded synthetic system, runs the reference's own TwoNO1
+C     (misc.f:166) and then either
+C       grid  : ACEInteg (acfd.f:4832) at each alpha node read from
+C               inputs.bin, printing W(alpha), ECASSCF and ERPA
+C               eigenvalue summaries at full precision;
+C       accas : the reference's ACCAS (accas.f:2) end to end, with its own
+C               pair selection and 5-point GauLeg (acfd.f:120);
+C       dumpab: AB_CAS (interpa.f:3659) at the first alpha node, writing
+C               ABPLUS/ABMIN to abplus.bin/abmin.bin.
+C     Pair selection for grid/dumpab mirrors accas.f:30-99 with
+C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
+      Program RefDrv
+      use types
+      Implicit Real*8 (A-H,O-Z)
+      Include 'commons.inc'
+      Character*60 Title
+      Character*16 Mode
+      Real*8, Allocatable :: Occ(:),XOne(:),UAOMO(:,:),URe(:,:),
+     $ TwoNO(:),AG(:),WG(:),ABPLUS(:),ABMIN(:),EigVecR(:),Eig(:),
+     $ EGOne(:),UNOAO(:,:)
+      Integer, Allocatable :: IndN(:,:),IndX(:),IndAux(:)
+      Integer Hdr(5)
+      Integer(8) NAddr3
+      External NAddr3
+      type(SystemBlock) :: System
+C
+      Call Get_Command_Argument(1,Mode)
+      Title='synthetic'
+      Open(11,File='inputs.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='OLD')
+      Read(11) Hdr
+      NBasis=Hdr(1)
+      NInAc=Hdr(2)
+      NAct=Hdr(3)
+      NEL=Hdr(4)
+      NAl=Hdr(5)
+      NInte1=NBasis*(NBasis+1)/2
+      NInte2=NInte1*(NInte1+1)/2
+      Allocate(Occ(NBasis),XOne(NInte1),UAOMO(NBasis,NBasis),
+     $ URe(NBasis,NBasis),TwoNO(NInte2),AG(NAl),WG(NAl),
+     $ UNOAO(NBasis,NBasis))
+      Read(11) Occ
+      Read(11) XOne
+      Read(11) UAOMO
+      Read(11) AG
+      Read(11) WG
+      Close(11)
+      Open(12,File='twoel_ao.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='OLD')
+      Read(12) TwoNO
+      Close(12)
+C
+C     COMMON BLOCKS
+C
+      XELE=Float(NEL)
+      NELE=NEL
+      IFunSR=0
+      IFunSRKer=0
+      ITwoEl=1
+      ICASSCF=1
+      NAcCAS=NAct
+      NInAcCAS=NInAc
+      NStates=1
+      IFlAC=1
+      IFlSnd=0
+      IFlAC0D=0
+      IFlAC0DP=0
+      IFlCore=1
+      IFlFrag1=0
+      IFl12=0
+      NoSt=1
+      IFlACFREQ=0
+      ThrSelAct=1.D-8
+      ThrAct=1.D-8
+      ThrQVirt=1.D-7
+      NGem=3
+      Do I=1,NBasis
+      IGem(I)=3
+      If(I.Le.NInAc+NAct) IGem(I)=2
+      If(I.Le.NInAc) IGem(I)=1
+      EndDo
+      If(NInAc.Eq.0) Then
+      NGem=2
+      Do I=1,NBasis
+      IGem(I)=2
+      If(I.Le.NAct) IGem(I)=1
+      EndDo
+      EndIf
+C
+      Call TwoNO1(TwoNO,UAOMO,NBasis,NInte2)
+      S1=0.D0
+      S2=0.D0
+      Do I=1,NInte2
+      S1=S1+TwoNO(I)
+      S2=S2+TwoNO(I)**2
+      EndDo
+      Write(6,'(A,2ES25.17)') 'TWONO_SUMS ',S1,S2
+      Do I=1,NBasis,7
+      J=Max(1,I/2)
+      K=Min(NBasis,I+3)
+      L=Max(1,K-5)
+      Write(6,'(A,4I6,ES25.17)') 'TWONO ',I,J,K,L,
+     $ TwoNO(NAddr3(I,J,K,L))
+      EndDo
+C
+      Do I=1,NBasis
+      Do J=1,NBasis
+      URe(I,J)=0.D0
+      UNOAO(I,J)=UAOMO(I,J)
+      EndDo
+      URe(I,I)=1.D0
+      EndDo
+C
+      If(Mode(1:5).Eq.'accas') Then
+      ENuc=0.D0
+      Call ACCAS(ETot,ENuc,TwoNO,URe,UNOAO,Occ,XOne,
+     $ Title,NBasis,NInte1,NInte2,NGem,System)
+      Stop
+      EndIf
+C
+C     PAIR SELECTION (accas.f:30-99)
+C
+      NDim=NBasis*(NBasis-1)/2
+      Allocate(IndN(2,NDim),IndX(NDim),IndAux(NBasis))
+      Do I=1,NBasis
+      IndAux(I)=2
+      If(I.Le.NELE) IndAux(I)=0
+      If(Occ(I).Lt.1.D0.And.Occ(I).Ne.0.D0) IndAux(I)=1
+      EndDo
+      Ind=0
+      Do I=1,NBasis
+      Do J=1,I-1
+      IS=IndAux(I)+IndAux(J)
+      If(IS.Ne.0.And.IS.Ne.4) Then
+      If(IndAux(I).Eq.1.And.IndAux(J).Eq.1.And.
+     $ Abs(Occ(I)-Occ(J))/Occ(I).Lt.ThrSelAct) Then
+      Continue
+      Else
+      If(Abs(Occ(I)+Occ(J)-2.D0).Gt.1.D-10.And.
+     $ Abs(Occ(I)+Occ(J)).Gt.ThrQVirt) Then
+      Ind=Ind+1
+      IndX(Ind)=Ind
+      IndN(1,Ind)=I
+      IndN(2,Ind)=J
+      EndIf
+      EndIf
+      EndIf
+      EndDo
+      EndDo
+      NDimX=Ind
+      Write(6,'(A,I8)') 'NDIMX ',NDimX
+C
+      Allocate(ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),
+     $ EigVecR(NDimX*NDimX),Eig(NDimX),EGOne(NGem))
+C
+      If(Mode(1:6).Eq.'dumpab') Then
+      Call AB_CAS(ABPLUS,ABMIN,ECASSCF,URe,Occ,XOne,TwoNO,IndAux,
+     $ IndN,IndX,NDimX,NBasis,NDimX,NInte1,NInte2,AG(1))
+      Write(6,'(A,ES25.17)') 'ECASSCF ',ECASSCF
+      Open(13,File='abplus.bin',Access='STREAM',Form='UNFORMATTED')
+      Write(13) ABPLUS
+      Close(13)
+      Open(13,File='abmin.bin',Access='STREAM',Form='UNFORMATTED')
+      Write(13) ABMIN
+      Close(13)
+      Stop
+      EndIf
+C
+      ECorr=0.D0
+      Do IA=1,NAl
+      Call ACEInteg(ECorrA,TwoNO,URe,Occ,XOne,UNOAO,
+     $ ABPLUS,ABMIN,EigVecR,Eig,
+     $ EGOne,NGOcc,
+     $ Title,NBasis,NInte1,NInte2,NDimX,NGem,IndAux,AG(IA),
+     $ IndN,IndX,NDimX)
+      ECorr=ECorr+WG(IA)*ECorrA
+      ES=0.D0
+      NPos=0
+      Do K=1,NDimX
+      If(Eig(K).Gt.1.D-3.And.Eig(K).Lt.1.D8) Then
+      ES=ES+Eig(K)
+      NPos=NPos+1
+      EndIf
+      EndDo
+      Write(6,'(A,I4,3ES25.17)') 'WALPHA ',IA,AG(IA),WG(IA),ECorrA
+      Write(6,'(A,I4,I8,4ES25.17)') 'EIG ',IA,NPos,ES,Eig(1),
+     $ Eig(NDimX/2),Eig(NDimX)
+      Write(6,'(A,ES25.17)') 'ECASSCF ',EGOne(1)
+      EndDo
+      Write(6,'(A,ES25.17)') 'ECORR ',ECorr
+C
+      End
