@@ -1,0 +1,63 @@
+"""ctypes binding of libgammcor_hip.so.
+
+The library is the product path: if it is missing or fails to load this
+module raises, there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgammcor_hip.so")
+
+_c_int, _c_i64, _c_double, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+# name -> argtypes (all functions return int status unless noted)
+SIGNATURES = {
+    "gmc_version": [],
+    "gmc_init": [],
+    "gmc_finalize": [],
+    "gmc_twono1": [_vp, _vp, _c_int, _c_i64],
+    "gmc_twono1_dev": [_vp, _vp, _c_int, _c_i64, _vp],
+    "gmc_gemm_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
+                     _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
+}
+
+_lib = None
+
+
+class GmcError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GmcError(f"{LIB_PATH} is not built; run `python -m gammcor_amd.build` "
+                           "(there is no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = _c_int
+        l.gmc_last_error.argtypes = []
+        l.gmc_last_error.restype = ctypes.c_char_p
+        _lib = l
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().gmc_last_error().decode(errors="replace")
+        raise GmcError(f"gammcor_hip status {rc}: {msg}")
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def exported_symbols():
+    return list(SIGNATURES) + ["gmc_last_error"]

@@ -1,0 +1,76 @@
+// Shared definitions for the gfx950 AC-CAS library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace gmc {
+
+// Last error message of the calling thread (returned by gmc_last_error()).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+struct Status {
+    int code = 0;
+};
+
+#define GMC_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::gmc::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,                 \
+                             hipGetErrorString(e_));                                      \
+            return -1;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+#define GMC_CHECK(cond, ...)                                                              \
+    do {                                                                                  \
+        if (!(cond)) {                                                                    \
+            ::gmc::set_error(__VA_ARGS__);                                                \
+            return -2;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+#define GMC_TRY(expr)                                                                     \
+    do {                                                                                  \
+        int rc_ = (expr);                                                                 \
+        if (rc_ != 0) return rc_;                                                         \
+    } while (0)
+
+// Device scratch owned by the library: grows on demand, freed at finalize.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t need);
+    template <class T>
+    T* as(size_t offset_bytes = 0) const { return reinterpret_cast<T*>(static_cast<char*>(ptr) + offset_bytes); }
+};
+
+inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
+
+// Sub-allocator over a Workspace, used to carve per-call scratch.
+struct Arena {
+    char* base;
+    size_t cap;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        size_t o = align_up(off);
+        off = o + count * sizeof(T);
+        return reinterpret_cast<T*>(base + o);
+    }
+};
+
+inline int64_t pair_idx(int64_t i, int64_t j) {
+    int64_t a = i > j ? i : j, b = i > j ? j : i;
+    return a * (a + 1) / 2 + b;
+}
+
+__host__ __device__ inline int64_t pidx(int64_t i, int64_t j) {
+    int64_t a = i > j ? i : j, b = i > j ? j : i;
+    return a * (a + 1) / 2 + b;
+}
+
+}  // namespace gmc

@@ -1,0 +1,28 @@
+// Batched column-major GEMM on MFMA (gfx950):
+//   C_b = alpha * op(A_b) * op(B_b) + beta * C_b,   b = 0..batch-1
+// FP64 uses v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32.
+#pragma once
+#include "common.hpp"
+
+namespace gmc {
+
+struct GemmShape {
+    int m, n, k;
+    bool ta, tb;          // op(X) = X^T when set
+    int64_t lda, ldb, ldc;
+    int64_t sa, sb, sc;   // batch strides in elements
+    int batch;
+    // Restrict the written part of C: 0 = full, 1 = lower triangle (i >= j)
+    // at tile granularity plus an exact element mask.
+    int uplo = 0;
+};
+
+template <class T>
+int gemm(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
+
+// Same, with one pointer per batch entry (device arrays of pointers).
+template <class T>
+int gemm_ptr(hipStream_t s, const GemmShape& g, T alpha, const T* const* A, const T* const* B, T beta,
+             T* const* C);
+
+}  // namespace gmc

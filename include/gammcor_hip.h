@@ -1,0 +1,49 @@
+/*
+ * gammcor_hip.h - C ABI of the MI355X (gfx950) AC-CAS library.
+ *
+ * Conventions shared by every entry point:
+ *   - return 0 on success, a negative status on failure; gmc_last_error()
+ *     returns the message of the calling thread's last failure;
+ *   - matrices are column-major (Fortran order), index arrays are 1-based
+ *     exactly as the reference passes them;
+ *   - "_dev" entry points take device pointers and an optional HIP stream
+ *     (NULL = the library stream, synchronised before return); the others
+ *     take host pointers and are synchronous;
+ *   - the library is not re-entrant (neither are the reference routines).
+ */
+#ifndef GAMMCOR_HIP_H
+#define GAMMCOR_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMC_ABI_VERSION 1
+#define GMC_FP64 0
+#define GMC_FP32 1
+
+const char* gmc_last_error(void);
+int gmc_version(void);
+int gmc_init(void);
+int gmc_finalize(void);
+
+/* TwoNO1(TNO, URe, NBasis, NInte2) - misc.f:166.  In-place 4-index
+ * transformation of the 8-fold packed two-electron integrals (NAddr3 order,
+ * misc.f:505) with URe(NBasis,NBasis) = UAOMO(NO,AO).  int64 NInte2 lifts the
+ * reference's 32-bit limit. */
+int gmc_twono1(double* tno, const double* ure, int nbasis, int64_t ninte2);
+int gmc_twono1_dev(double* tno, const double* ure, int nbasis, int64_t ninte2, void* stream);
+
+/* Building block exported for tests: batched column-major GEMM on MFMA,
+ * C_b = alpha op(A_b) op(B_b) + beta C_b.  prec = GMC_FP64 or GMC_FP32 (then
+ * the pointers are float and alpha/beta are rounded to float).  No
+ * counterpart in the reference (it calls DGEMM, matvec.f:148). */
+int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, const void* A, int64_t lda,
+                 int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
+                 int64_t sc, int batch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
