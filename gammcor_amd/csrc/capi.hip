@@ -41,6 +41,25 @@ State& state()
     return st;
 }
 
+template <class T>
+int State::eig_plan_for(int n, int batch, EigPlan<T>** out)
+{
+    auto& cache = [this]() -> auto& {
+        if constexpr (sizeof(T) == 8) return eig64; else return eig32;
+    }();
+    auto key = std::make_pair(n, batch);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        auto p = std::make_unique<EigPlan<T>>();
+        GMC_TRY(eig_plan<T>(n, batch, p.get()));
+        it = cache.emplace(key, std::move(p)).first;
+    }
+    *out = it->second.get();
+    return 0;
+}
+template int State::eig_plan_for<double>(int, int, EigPlan<double>**);
+template int State::eig_plan_for<float>(int, int, EigPlan<float>**);
+
 int ensure_init()
 {
     State& st = state();
@@ -57,6 +76,20 @@ int ensure_init()
 }  // namespace gmc
 
 using namespace gmc;
+
+template <class T>
+static int syevd_dev_impl(int n, int batch, T* A, T* w, hipStream_t s)
+{
+    State& st = state();
+    EigPlan<T>* p = nullptr;
+    GMC_TRY(st.eig_plan_for<T>(n, batch, &p));
+    GMC_TRY(st.scratch.reserve(p->ws_bytes));
+    GMC_TRY(eig_bind<T>(p, st.scratch.ptr));
+    GMC_TRY(syevd_batched<T>(s, *p, A, w));
+    GMC_TRY(eig_check_error<T>(s, *p));
+    return 0;
+}
+
 
 extern "C" {
 
@@ -135,6 +168,14 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
         GMC_TRY(gemm<float>(s, g, (float)alpha, (const float*)A, (const float*)B, (float)beta, (float*)C));
     if (!stream) GMC_HIP(hipStreamSynchronize(s));
     return 0;
+}
+
+int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream)
+{
+    GMC_TRY(ensure_init());
+    hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    if (prec == GMC_FP64) return syevd_dev_impl<double>(n, batch, (double*)A, (double*)w, s);
+    return syevd_dev_impl<float>(n, batch, (float*)A, (float*)w, s);
 }
 
 }  // extern "C"

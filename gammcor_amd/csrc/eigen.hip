@@ -1,0 +1,1022 @@
+// Batched symmetric eigensolver (see eigen.hpp for the structure).
+#include <algorithm>
+#include <cmath>
+#include <functional>
+
+#include "eigen.hpp"
+#include "gemm_tile.hpp"
+
+namespace gmc {
+
+template <class T> struct Eps;
+template <> struct Eps<double> { static constexpr double v = 2.220446049250313e-16; };
+template <> struct Eps<float> { static constexpr float v = 1.1920929e-07f; };
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline float wave_sum(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ inline T wave_prod(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v *= __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T>
+__device__ inline T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum over the 256 threads of a workgroup; every thread gets the result.
+template <class T>
+__device__ T block_sum(T v, T* red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+template <class T>
+__device__ T block_max(T v, T* red)
+{
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Barrier among the G workgroups that cooperate on one problem.  A bounded
+// spin turns a lost arrival into an error flag instead of a hang.
+__device__ inline void group_barrier(unsigned* ctr, unsigned G, unsigned& epoch, int* err)
+{
+    __syncthreads();
+    epoch += 1;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(ctr, 1u);
+        const unsigned target = epoch * G;
+        long spins = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (++spins > (1L << 26)) {
+                atomicExch(err, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+__global__ void zero_u32(unsigned* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ====================================================================== sytrd
+template <class T>
+struct PanelArgs {
+    T *A, *W, *X, *d, *e, *tau, *part;
+    unsigned* bar;
+    int* err;
+    int n, k, kb, nb, G;
+    unsigned epoch0;
+};
+
+// Rows owned by workgroup gi: 64-row blocks gi, gi+G, ...; slot q -> row.
+__device__ inline int owned_row(int q, int gi, int G) { return 64 * (gi + G * (q >> 6)) + (q & 63); }
+
+template <class T>
+__global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
+{
+    extern __shared__ char smem[];
+    const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
+    const int p = blockIdx.x / G, gi = blockIdx.x % G;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (n + 63) / 64;
+    const int nslots = 64 * ((nblk - gi + G - 1) / G);
+
+    T* A = g.A + (size_t)p * n * n;
+    T* W = g.W + (size_t)p * n * nb;
+    T* X = g.X + (size_t)p * n * 3 * nb;
+    T* part = g.part + (size_t)p * (1 + G * (2 + 2 * nb));
+    T* p_alpha = part;
+    T* p_norm = part + 1;
+    T* p_dot = part + 1 + G;
+    T* p_wv = part + 1 + G + 2 * nb * G;
+    unsigned* bar = g.bar + p * 16;
+    int* err = g.err;
+
+    T* v = reinterpret_cast<T*>(smem);             // n
+    T* y = v + n;                                   // nslots
+    T* Wj = y + nslots;                             // nb
+    T* Aj = Wj + nb;
+    T* a2 = Aj + nb;
+    T* t12 = a2 + nb;                               // 2 nb
+    T* red = t12 + 2 * nb;                          // 8
+    T* sc = red + 8;                                // [0]=tau [1]=scal
+
+    unsigned epoch = g.epoch0;
+    T tau_prev = 0;
+    for (int i = 0; i < kb; i++) {
+        const int j = k + i;
+        if (i > 0 && tid == 0) {
+            T s = 0;
+            for (int q = 0; q < G; q++) s += p_wv[q];
+            a2[i - 1] = T(-0.5) * tau_prev * s;
+        }
+        __syncthreads();
+        for (int c = tid; c < i; c += 256) {
+            T ajc = A[j + (size_t)(k + c) * n];
+            Aj[c] = ajc;
+            Wj[c] = W[j + (size_t)c * n] + a2[c] * ajc;
+        }
+        __syncthreads();
+        // step 1: update column j with the panel so far; norms
+        T nrm = 0;
+        for (int q = tid; q < nslots; q += 256) {
+            const int r = owned_row(q, gi, G);
+            if (r < j || r >= n) continue;
+            T a = A[r + (size_t)j * n];
+            if (i > 0) {
+                for (int c = 0; c < i; c++) {
+                    T arc = A[r + (size_t)(k + c) * n];
+                    a -= arc * Wj[c] + (W[r + (size_t)c * n] + a2[c] * arc) * Aj[c];
+                }
+                A[r + (size_t)j * n] = a;
+            }
+            if (r == j) g.d[(size_t)p * n + j] = a;
+            if (r == j + 1) *p_alpha = a;
+            if (r >= j + 2) nrm += a * a;
+        }
+        nrm = block_sum(nrm, red);
+        if (tid == 0) p_norm[gi] = nrm;
+        group_barrier(bar, G, epoch, err);
+
+        // step 2: reflector, v into LDS; step 3: SYMV and panel dots
+        if (tid == 0) {
+            T alpha = *p_alpha, xn2 = 0;
+            for (int q = 0; q < G; q++) xn2 += p_norm[q];
+            T tau, beta, scal;
+            if (xn2 == T(0)) {
+                tau = 0;
+                beta = alpha;
+                scal = 0;
+            } else {
+                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                tau = (beta - alpha) / beta;
+                scal = T(1) / (alpha - beta);
+            }
+            sc[0] = tau;
+            sc[1] = scal;
+            if (gi == 0) {
+                g.tau[(size_t)p * n + j] = tau;
+                g.e[(size_t)p * n + j] = beta;
+            }
+        }
+        __syncthreads();
+        const T tau = sc[0], scal = sc[1];
+        for (int c = j + 1 + tid; c < n; c += 256) v[c] = (c == j + 1) ? T(1) : scal * A[c + (size_t)j * n];
+        __syncthreads();
+        for (int q = wave; q < nslots; q += 4) {
+            const int r = owned_row(q, gi, G);
+            if (r <= j || r >= n) continue;
+            const T* col = A + (size_t)r * n;
+            T s = 0;
+            for (int c = j + 1 + lane; c < n; c += 64) s += col[c] * v[c];
+            s = wave_sum(s);
+            if (lane == 0) y[q] = s;
+        }
+        for (int c = wave; c < 2 * i; c += 4) {
+            const int cc = c < i ? c : c - i;
+            T s = 0;
+            for (int q = lane; q < nslots; q += 64) {
+                const int r = owned_row(q, gi, G);
+                if (r <= j || r >= n) continue;
+                T arc = A[r + (size_t)(k + cc) * n];
+                T x = c < i ? W[r + (size_t)cc * n] + a2[cc] * arc : arc;
+                s += x * v[r];
+            }
+            s = wave_sum(s);
+            if (lane == 0) p_dot[gi * 2 * nb + c] = s;
+        }
+        group_barrier(bar, G, epoch, err);
+
+        // step 4: w = tau (y - V t1 - W t2), w.v partial, scaled v into A
+        for (int c = tid; c < 2 * i; c += 256) {
+            T s = 0;
+            for (int q = 0; q < G; q++) s += p_dot[q * 2 * nb + c];
+            t12[c] = s;
+        }
+        __syncthreads();
+        T wv = 0;
+        for (int q = tid; q < nslots; q += 256) {
+            const int r = owned_row(q, gi, G);
+            if (r <= j || r >= n) continue;
+            T yy = y[q];
+            for (int c = 0; c < i; c++) {
+                T arc = A[r + (size_t)(k + c) * n];
+                yy -= arc * t12[c] + (W[r + (size_t)c * n] + a2[c] * arc) * t12[i + c];
+            }
+            T wr = tau * yy;
+            W[r + (size_t)i * n] = wr;
+            wv += wr * v[r];
+            A[r + (size_t)j * n] = v[r];
+        }
+        wv = block_sum(wv, red);
+        if (tid == 0) p_wv[gi] = wv;
+        tau_prev = tau;
+        group_barrier(bar, G, epoch, err);
+    }
+    if (tid == 0) {
+        T s = 0;
+        for (int q = 0; q < G; q++) s += p_wv[q];
+        a2[kb - 1] = T(-0.5) * tau_prev * s;
+    }
+    __syncthreads();
+    // X = [V | W | V] for the trailing rows owned here
+    const int k1 = k + kb;
+    for (int q = tid; q < nslots; q += 256) {
+        const int r = owned_row(q, gi, G);
+        if (r < k1 || r >= n) continue;
+        for (int c = 0; c < kb; c++) {
+            T vr = A[r + (size_t)(k + c) * n];
+            T wr = W[r + (size_t)c * n] + a2[c] * vr;
+            X[r + (size_t)c * n] = vr;
+            X[r + (size_t)(kb + c) * n] = wr;
+            X[r + (size_t)(2 * kb + c) * n] = vr;
+        }
+    }
+}
+
+// After the last panel: d[n-1], and clear everything on/above the diagonal so
+// the strictly lower triangle holds the unit-lower reflectors.
+template <class T>
+__global__ void sytrd_finish_kernel(const T* A, T* d, int n, int B)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < B) d[(size_t)p * n + n - 1] = A[(size_t)p * n * n + (size_t)(n - 1) * n + (n - 1)];
+}
+
+template <class T>
+__global__ void clear_upper_kernel(T* A, int n)
+{
+    const int p = blockIdx.y;
+    T* a = A + (size_t)p * n * n;
+    const int64_t total = (int64_t)n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int r = (int)(idx % n), c = (int)(idx / n);
+        if (r <= c) a[idx] = T(0);
+    }
+}
+
+// ====================================================================== ormtr
+// T factor (upper triangular, ld nbb) of H(k)...H(k+kb-1) = I - V T V^T from
+// the Gram matrix Gm = V^T V (ld nbb) and tau.
+template <class T>
+__global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int k, int kb, int nbb)
+{
+    const int p = blockIdx.x;
+    const T* G = Gm + (size_t)p * nbb * nbb;
+    T* Tm = Tf + (size_t)p * nbb * nbb;
+    const T* ta = tau + (size_t)p * n + k;
+    __shared__ T tmp[256];
+    for (int idx = threadIdx.x; idx < nbb * nbb; idx += 256) Tm[idx] = T(0);
+    __syncthreads();
+    for (int i = 0; i < kb; i++) {
+        const T ti = ta[i];
+        if (threadIdx.x < i) tmp[threadIdx.x] = G[threadIdx.x + (size_t)i * nbb];
+        __syncthreads();
+        if (threadIdx.x < i) {
+            const int c = threadIdx.x;
+            T s = 0;
+            for (int c2 = c; c2 < i; c2++) s += Tm[c + (size_t)c2 * nbb] * tmp[c2];
+            Tm[c + (size_t)i * nbb] = -ti * s;
+        }
+        if (threadIdx.x == 0) Tm[i + (size_t)i * nbb] = ti;
+        __syncthreads();
+    }
+}
+
+// ====================================================================== D&C
+template <class T>
+__global__ void dc_init_kernel(const T* d, const T* e, const unsigned char* split, T* lam, T* Q, int n)
+{
+    const int p = blockIdx.y;
+    const int64_t total = (int64_t)n * n;
+    T* q = Q + (size_t)p * n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        int r = (int)(idx % n), c = (int)(idx / n);
+        q[idx] = (r == c) ? T(1) : T(0);
+        if (r == 0) {
+            const T* dd = d + (size_t)p * n;
+            const T* ee = e + (size_t)p * n;
+            T v = dd[c];
+            if (c > 0 && split[c - 1]) v -= fabs(ee[c - 1]);
+            if (c < n - 1 && split[c]) v -= fabs(ee[c]);
+            lam[(size_t)p * n + c] = v;
+        }
+    }
+}
+
+struct DcArgs {
+    int n, nmerge, batch;
+    const int* desc;    // s, m1, m2, tile offset
+};
+
+template <class T>
+struct DcBufs {
+    const T* e;
+    T *lam, *Q, *QK, *QN, *UB;
+    T *dsort, *zsort, *dK, *zK, *taus, *zh, *EV, *rotc, *rots;
+    int *perm, *lst, *lstD, *org, *rank, *rotp, *rotj, *kcnt, *rcnt;
+};
+
+__device__ inline int find_merge(const int* desc, int nmerge, int x)
+{
+    int lo = 0, hi = nmerge - 1, ans = -1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (desc[4 * mid] <= x) {
+            ans = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    if (ans >= 0 && x < desc[4 * ans] + desc[4 * ans + 1] + desc[4 * ans + 2]) return ans;
+    return -1;
+}
+
+// z vector, sorted merge of the two children, deflation scan.
+template <class T>
+__global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
+{
+    __shared__ T red[8];
+    __shared__ T cd[256], cz[256];
+    const int n = a.n;
+    const int p = blockIdx.x / a.nmerge, q = blockIdx.x % a.nmerge;
+    const int s = a.desc[4 * q], m1 = a.desc[4 * q + 1], m2 = a.desc[4 * q + 2];
+    const int m = m1 + m2;
+    const size_t po = (size_t)p * n;
+    const T* Q = b.Q + po * n;
+    const T* lam = b.lam + po;
+    T* dsort = b.dsort + po + s;
+    T* zsort = b.zsort + po + s;
+    int* perm = b.perm + po + s;
+    const T beta = b.e[po + s + m1 - 1];
+    const T rho = T(2) * fabs(beta);
+    const T sgn = beta >= T(0) ? T(1) : T(-1);
+    const T rs2 = T(0.70710678118654752440);
+    T dmax = 0, zmax = 0;
+    for (int t = threadIdx.x; t < m; t += 256) {
+        T D = lam[s + t], z;
+        int pos;
+        if (t < m1) {
+            z = Q[(s + m1 - 1) + (size_t)(s + t) * n] * rs2;
+            int lo = 0, hi = m2;   // count right values < D
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (lam[s + m1 + mid] < D) lo = mid + 1; else hi = mid;
+            }
+            pos = t + lo;
+        } else {
+            z = sgn * Q[(s + m1) + (size_t)(s + t) * n] * rs2;
+            int u = t - m1, lo = 0, hi = m1;   // count left values <= D
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (lam[s + mid] <= D) lo = mid + 1; else hi = mid;
+            }
+            pos = u + lo;
+        }
+        dsort[pos] = D;
+        zsort[pos] = z;
+        perm[pos] = t;
+        dmax = fmax(dmax, fabs(D));
+        zmax = fmax(zmax, fabs(z));
+    }
+    dmax = block_max(dmax, red);
+    zmax = block_max(zmax, red);
+    const T tol = T(8) * Eps<T>::v * fmax(dmax, zmax);
+    __syncthreads();
+
+    // sequential deflation scan (thread 0), m values staged 256 at a time
+    int k = 0, ndf = 0, nrot = 0, pj = -1;
+    T dpj = 0, zpj = 0;
+    int* lst = b.lst + po + s;
+    int* lstD = b.lstD + po + s;
+    T* dK = b.dK + po + s;
+    T* zK = b.zK + po + s;
+    for (int c0 = 0; c0 < m; c0 += 256) {
+        __syncthreads();
+        if (c0 + (int)threadIdx.x < m) {
+            cd[threadIdx.x] = dsort[c0 + threadIdx.x];
+            cz[threadIdx.x] = zsort[c0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int ce = min(256, m - c0);
+            for (int jj = 0; jj < ce; jj++) {
+                const int j = c0 + jj;
+                T dj = cd[jj], zj = cz[jj];
+                if (rho * fabs(zj) <= tol) {
+                    lstD[ndf++] = j;
+                    continue;
+                }
+                if (pj < 0) {
+                    pj = j;
+                    dpj = dj;
+                    zpj = zj;
+                    continue;
+                }
+                T sv = zpj, cv = zj;
+                T tv = hypot(cv, sv);
+                T t = dj - dpj;
+                cv /= tv;
+                sv = -sv / tv;
+                if (fabs(t * cv * sv) <= tol) {
+                    b.rotp[po + s + nrot] = pj;
+                    b.rotj[po + s + nrot] = j;
+                    b.rotc[po + s + nrot] = cv;
+                    b.rots[po + s + nrot] = sv;
+                    nrot++;
+                    T t2 = dpj * cv * cv + dj * sv * sv;
+                    T djn = dpj * sv * sv + dj * cv * cv;
+                    dsort[pj] = t2;
+                    lstD[ndf++] = pj;
+                    pj = j;
+                    dpj = djn;
+                    zpj = tv;
+                    dsort[j] = djn;
+                } else {
+                    lst[k] = pj;
+                    dK[k] = dpj;
+                    zK[k] = zpj;
+                    k++;
+                    pj = j;
+                    dpj = dj;
+                    zpj = zj;
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (pj >= 0) {
+            lst[k] = pj;
+            dK[k] = dpj;
+            zK[k] = zpj;
+            k++;
+        }
+        b.kcnt[po + q] = k;
+        b.rcnt[po + q] = nrot;
+    }
+}
+
+// Apply the deflation rotations to the merge's columns of Q (one row per thread).
+template <class T>
+__global__ __launch_bounds__(256) void dc_rot_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int p = blockIdx.y, q = blockIdx.x;
+    const size_t po = (size_t)p * n;
+    const int nrot = b.rcnt[po + q];
+    if (nrot == 0) return;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    T* Q = b.Q + po * n;
+    const int* perm = b.perm + po + s;
+    for (int r = threadIdx.x; r < m; r += 256) {
+        for (int t = 0; t < nrot; t++) {
+            const int cx = s + perm[b.rotp[po + s + t]], cy = s + perm[b.rotj[po + s + t]];
+            const T c = b.rotc[po + s + t], sn = b.rots[po + s + t];
+            T* x = Q + (s + r) + (size_t)cx * n;
+            T* y = Q + (s + r) + (size_t)cy * n;
+            T xv = *x, yv = *y;
+            *x = c * xv + sn * yv;
+            *y = c * yv - sn * xv;
+        }
+    }
+}
+
+// Secular equation 1/rho + sum z_j^2/(d_j - lam) = 0, one wave per root.
+template <class T>
+__global__ __launch_bounds__(256) void dc_secular_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int p = (int)(wid / n), x = (int)(wid % n);
+    if (p >= a.batch) return;   // the grid is rounded up to whole workgroups
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q];
+    const int k = b.kcnt[po + q];
+    const int i = x - s;
+    if (i >= k) return;
+    const T* dK = b.dK + po + s;
+    const T* zK = b.zK + po + s;
+    const T beta = b.e[po + s + a.desc[4 * q + 1] - 1];
+    const T rho = T(2) * fabs(beta);
+    const T rinv = T(1) / rho;
+    int o;
+    T lo, hi;
+    if (i < k - 1) {
+        const T di = dK[i];
+        const T mid = T(0.5) * (dK[i + 1] - di);
+        T f = 0;
+        for (int j = lane; j < k; j += 64) {
+            T zj = zK[j];
+            f += zj * zj / ((dK[j] - di) - mid);
+        }
+        f = wave_sum(f) + rinv;
+        if (f >= T(0)) {
+            o = i;
+            lo = 0;
+            hi = mid;
+        } else {
+            o = i + 1;
+            lo = -mid;
+            hi = 0;
+        }
+    } else {
+        o = i;
+        lo = 0;
+        hi = rho;
+    }
+    const T dorg = dK[o];
+    const T Dl = dK[i] - dorg;
+    const T Dr = (i + 1 < k) ? dK[i + 1] - dorg : T(0);
+    const T zl2 = zK[i] * zK[i];
+    const T zr2 = (i + 1 < k) ? zK[i + 1] * zK[i + 1] : T(0);
+    const T eps = Eps<T>::v;
+    T t = T(0.5) * (lo + hi);
+    T wprev = hi - lo;
+    for (int it = 0; it < 200; it++) {
+        T gsum = 0, gabs = 0;
+        for (int j = lane; j < k; j += 64) {
+            T zj = zK[j];
+            T term = zj * zj / ((dK[j] - dorg) - t);
+            gsum += term;
+            gabs += fabs(term);
+        }
+        gsum = wave_sum(gsum);
+        gabs = wave_sum(gabs);
+        const T gv = gsum + rinv;
+        if (gv == T(0)) break;
+        if (gv < T(0)) lo = t; else hi = t;
+        if (fabs(gv) <= T(8) * eps * (rinv + gabs)) break;
+        if (hi - lo <= T(4) * eps * fmax(fabs(lo), fabs(hi))) break;
+        const bool force = (it & 1) && (hi - lo) > T(0.5) * wprev;
+        if (it & 1) wprev = hi - lo;
+        // fixed-weight rational step with the two bracketing poles
+        const T cres = gv - zl2 / (Dl - t) - ((i + 1 < k) ? zr2 / (Dr - t) : T(0));
+        T tn = T(0.5) * (lo + hi);
+        bool ok = false;
+        if (i + 1 < k) {
+            const T a2 = cres;
+            const T a1 = -(cres * (Dl + Dr) + zl2 + zr2);
+            const T a0 = cres * Dl * Dr + zl2 * Dr + zr2 * Dl;
+            if (a2 == T(0)) {
+                if (a1 != T(0)) {
+                    T c1 = -a0 / a1;
+                    if (c1 > lo && c1 < hi) { tn = c1; ok = true; }
+                }
+            } else {
+                T disc = a1 * a1 - T(4) * a2 * a0;
+                if (disc >= T(0)) {
+                    T sq = sqrt(disc);
+                    T qq = T(-0.5) * (a1 + copysign(sq, a1));
+                    T c1 = qq / a2;
+                    if (c1 > lo && c1 < hi) { tn = c1; ok = true; }
+                    else if (qq != T(0)) {
+                        T c2 = a0 / qq;
+                        if (c2 > lo && c2 < hi) { tn = c2; ok = true; }
+                    }
+                }
+            }
+        } else if (cres > T(0)) {
+            T c1 = Dl + zl2 / cres;
+            if (c1 > lo && c1 < hi) { tn = c1; ok = true; }
+        }
+        if (!ok || force) tn = T(0.5) * (lo + hi);
+        t = tn;
+    }
+    if (lane == 0) {
+        b.org[po + s + i] = o;
+        b.taus[po + s + i] = t;
+    }
+}
+
+// Gu-Eisenstat z-hat, one wave per component.
+template <class T>
+__global__ __launch_bounds__(256) void dc_zhat_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int p = (int)(wid / n), x = (int)(wid % n);
+    if (p >= a.batch) return;   // the grid is rounded up to whole workgroups
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q];
+    const int k = b.kcnt[po + q];
+    const int j = x - s;
+    if (j >= k) return;
+    const T* dK = b.dK + po + s;
+    const T* taus = b.taus + po + s;
+    const int* org = b.org + po + s;
+    const T rho = T(2) * fabs(b.e[po + s + a.desc[4 * q + 1] - 1]);
+    const T dj = dK[j];
+    T prod = T(1);
+    for (int i = lane; i < k; i += 64) {
+        const T lmj = (dK[org[i]] - dj) + taus[i];
+        T f;
+        if (i < j) f = lmj / (dK[i] - dj);
+        else if (i < k - 1) f = lmj / (dK[i + 1] - dj);
+        else f = lmj / rho;
+        prod *= f;
+    }
+    prod = wave_prod(prod);
+    if (lane == 0) b.zh[po + s + j] = copysign(sqrt(fmax(prod, T(0))), b.zK[po + s + j]);
+}
+
+// Rank-one eigenvectors U (k x k, ld n at column offset s) and gather of the
+// non-deflated columns of Q into QK; one wave per column.
+template <class T>
+__global__ __launch_bounds__(256) void dc_vec_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int p = (int)(wid / n), x = (int)(wid % n);
+    if (p >= a.batch) return;   // the grid is rounded up to whole workgroups
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    const int k = b.kcnt[po + q];
+    const int i = x - s;
+    if (i >= k) return;
+    const T* dK = b.dK + po + s;
+    const T* zh = b.zh + po + s;
+    const T dorg = dK[b.org[po + s + i]];
+    const T ti = b.taus[po + s + i];
+    T ss = 0;
+    for (int t = lane; t < k; t += 64) {
+        T u = zh[t] / ((dK[t] - dorg) - ti);
+        ss += u * u;
+    }
+    ss = wave_sum(ss);
+    const T inv = T(1) / sqrt(ss);
+    T* U = b.UB + po * n + (size_t)(s + i) * n;
+    for (int t = lane; t < k; t += 64) U[t] = zh[t] / ((dK[t] - dorg) - ti) * inv;
+    // gather: QK[:, s+i] = Q[s.., s + perm[lst[i]]]
+    const int col = s + b.perm[po + s + b.lst[po + s + i]];
+    const T* src = b.Q + po * n + s + (size_t)col * n;
+    T* dst = b.QK + po * n + (size_t)(s + i) * n;
+    for (int r = lane; r < m; r += 64) dst[r] = src[r];
+}
+
+// Grouped merge GEMM: QN[:, s..s+k) = QK[:, s..s+k) * U, per merge.
+template <class T>
+__global__ __launch_bounds__(256) void dc_gemm_kernel(DcArgs a, DcBufs<T> b, int64_t tiles)
+{
+    const int n = a.n;
+    const int p = (int)(blockIdx.x / tiles);
+    const int64_t t = blockIdx.x % tiles;
+    int lo = 0, hi = a.nmerge - 1, q = 0;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (a.desc[4 * mid + 3] <= t) { q = mid; lo = mid + 1; } else hi = mid - 1;
+    }
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    const int k = b.kcnt[po + q];
+    const int ntm = (m + 63) / 64;
+    const int64_t lt = t - a.desc[4 * q + 3];
+    const int ti = (int)(lt % ntm), tj = (int)(lt / ntm);
+    if (tj * 64 >= k || ti * 64 >= m) return;
+    GemmShape g{m, k, k, false, false, (int64_t)n, (int64_t)n, (int64_t)n, 0, 0, 0, 1, 0};
+    const T* A = b.QK + po * n + (size_t)s * n;
+    const T* B = b.UB + po * n + (size_t)s * n;
+    T* C = b.QN + po * n + (size_t)s * n;
+    gemm_tile<T, GemmBK<T>::value>(g, ti * 64, tj * 64, T(1), A, B, T(0), C);
+}
+
+// Eigenvalues of the merged block and the deflated columns appended to QN.
+template <class T>
+__global__ __launch_bounds__(256) void dc_final1_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int p = blockIdx.y, x = blockIdx.x;
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    const int k = b.kcnt[po + q];
+    const int t = x - s;
+    if (t < k) {
+        if (threadIdx.x == 0) b.EV[po + x] = b.dK[po + s + b.org[po + s + t]] + b.taus[po + s + t];
+        return;
+    }
+    const int pos = b.lstD[po + s + (t - k)];
+    if (threadIdx.x == 0) b.EV[po + x] = b.dsort[po + s + pos];
+    const int col = s + b.perm[po + s + pos];
+    const T* src = b.Q + po * n + s + (size_t)col * n;
+    T* dst = b.QN + po * n + (size_t)x * n;
+    for (int r = threadIdx.x; r < m; r += 256) dst[r] = src[r];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void dc_rank_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int p = blockIdx.y;
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= n) return;
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    const T* EV = b.EV + po + s;
+    const int t = x - s;
+    const T v = EV[t];
+    int r = 0;
+    for (int u = 0; u < m; u++) {
+        T w = EV[u];
+        r += (w < v || (w == v && u < t)) ? 1 : 0;
+    }
+    b.rank[po + x] = s + r;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void dc_final2_kernel(DcArgs a, DcBufs<T> b)
+{
+    const int n = a.n;
+    const int p = blockIdx.y, x = blockIdx.x;
+    const int q = find_merge(a.desc, a.nmerge, x);
+    if (q < 0) return;
+    const size_t po = (size_t)p * n;
+    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
+    const int dst_col = b.rank[po + x];
+    if (threadIdx.x == 0) b.lam[po + dst_col] = b.EV[po + x];
+    const T* src = b.QN + po * n + (size_t)x * n;
+    T* dst = b.Q + po * n + s + (size_t)dst_col * n;
+    for (int r = threadIdx.x; r < m; r += 256) dst[r] = src[r];
+}
+
+template <class T>
+__global__ void copy_kernel(const T* src, T* dst, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ====================================================================== plan
+template <class T>
+EigPlan<T>::~EigPlan()
+{
+    for (auto& l : levels)
+        if (l.d_desc) (void)hipFree(l.d_desc);
+    if (d_split) (void)hipFree(d_split);
+}
+
+// Carve the workspace; with base == nullptr only ws_bytes is computed.
+template <class T>
+static int eig_layout(EigPlan<T>* p, char* base)
+{
+    const size_t B = p->batch, n = p->n, nn = n * n, nb = p->nb, nbb = p->nbb, G = p->G;
+    size_t off = 0;
+    auto take = [&](auto*& ptr, size_t bytes) {
+        size_t o = align_up(off);
+        off = o + bytes;
+        ptr = base ? reinterpret_cast<std::remove_reference_t<decltype(ptr)>>(base + o) : nullptr;
+    };
+    take(p->W, B * n * nb * sizeof(T));
+    take(p->X, B * n * 3 * nb * sizeof(T));
+    take(p->d, B * n * sizeof(T));
+    take(p->e, B * n * sizeof(T));
+    take(p->tau, B * n * sizeof(T));
+    take(p->part, B * (1 + G * (2 + 2 * nb)) * sizeof(T));
+    take(p->Q, B * nn * sizeof(T));
+    take(p->QK, B * nn * sizeof(T));
+    take(p->QN, B * nn * sizeof(T));
+    take(p->UB, B * nn * sizeof(T));
+    for (T** v : {&p->dsort, &p->zsort, &p->dK, &p->zK, &p->taus, &p->zh, &p->EV, &p->rotc, &p->rots})
+        take(*v, B * n * sizeof(T));
+    for (int** v : {&p->perm, &p->lst, &p->lstD, &p->org, &p->rank, &p->rotp, &p->rotj, &p->kcnt, &p->rcnt})
+        take(*v, B * n * sizeof(int));
+    take(p->Tf, B * 2 * nbb * nbb * sizeof(T));
+    take(p->bar, (B + 1) * 16 * sizeof(unsigned));
+    take(p->err, 64);
+    p->ws_bytes = align_up(off);
+    return 0;
+}
+
+template <class T>
+int eig_plan(int n, int batch, EigPlan<T>* p)
+{
+    GMC_CHECK(n >= 1 && batch >= 1, "eig_plan: bad sizes n=%d batch=%d", n, batch);
+    p->n = n;
+    p->batch = batch;
+    // workgroups per problem: all problems resident at once (one per CU)
+    int ncu = 256;
+    {
+        hipDeviceProp_t prop;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            ncu = prop.multiProcessorCount;
+    }
+    const int nblk = (n + 63) / 64;
+    p->G = std::max(1, std::min(nblk, ncu / batch));
+    GMC_CHECK(batch <= ncu, "eig_plan: batch %d exceeds the %d co-resident problems", batch, ncu);
+
+    // divide & conquer tree: recursive halving down to single rows
+    struct Node { int s, m1, m2, h; };
+    std::vector<Node> nodes;
+    std::vector<unsigned char> split(n > 1 ? n - 1 : 1, 0);
+    std::function<int(int, int)> build = [&](int s, int m) -> int {
+        if (m == 1) return 0;
+        int m1 = m / 2;
+        split[s + m1 - 1] = 1;
+        int h = 1 + std::max(build(s, m1), build(s + m1, m - m1));
+        nodes.push_back({s, m1, m - m1, h});
+        return h;
+    };
+    int H = build(0, n);
+    p->levels.assign(H, DcLevel{});
+    for (int h = 1; h <= H; h++) {
+        std::vector<Node> lv;
+        for (auto& nd : nodes)
+            if (nd.h == h) lv.push_back(nd);
+        std::sort(lv.begin(), lv.end(), [](const Node& x, const Node& y) { return x.s < y.s; });
+        std::vector<int> desc;
+        int64_t off = 0;
+        int maxm = 0;
+        for (auto& nd : lv) {
+            int m = nd.m1 + nd.m2;
+            desc.insert(desc.end(), {nd.s, nd.m1, nd.m2, (int)off});
+            int64_t tm = (m + 63) / 64;
+            off += tm * tm;
+            maxm = std::max(maxm, m);
+        }
+        GMC_CHECK(off < (1LL << 31), "eig_plan: tile count overflow");
+        DcLevel& L = p->levels[h - 1];
+        L.nmerge = (int)lv.size();
+        L.tiles = off;
+        L.maxm = maxm;
+        GMC_HIP(hipMalloc(&L.d_desc, desc.size() * sizeof(int)));
+        GMC_HIP(hipMemcpy(L.d_desc, desc.data(), desc.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    GMC_HIP(hipMalloc(&p->d_split, split.size()));
+    GMC_HIP(hipMemcpy(p->d_split, split.data(), split.size(), hipMemcpyHostToDevice));
+
+    p->nbb = std::min(p->nbb, n);
+    GMC_TRY(eig_layout(p, nullptr));
+    return 0;
+}
+
+template <class T>
+int eig_bind(EigPlan<T>* p, void* ws)
+{
+    GMC_TRY(eig_layout(p, static_cast<char*>(ws)));
+    p->bound = ws != nullptr;
+    return 0;
+}
+
+static inline unsigned grid1(int64_t work, int per = 256, int64_t cap = 1 << 20)
+{
+    int64_t g = (work + per - 1) / per;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+template <class T>
+int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
+{
+    GMC_CHECK(p.bound, "syevd: plan not bound to a workspace");
+    const int n = p.n, B = p.batch, nb = p.nb, G = p.G;
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(zero_u32, dim3((B * 16 + 16 + 255) / 256), dim3(256), 0, s, p.bar, B * 16 + 16);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, (unsigned*)p.err, 16);
+    GMC_HIP(hipGetLastError());
+
+    // ---------------- tridiagonalisation
+    if (n > 1) {
+        const int nblk = (n + 63) / 64;
+        const int nslots_max = 64 * ((nblk + G - 1) / G);
+        const size_t lds = (size_t)(n + nslots_max + 5 * nb + 16) * sizeof(T);
+        GMC_CHECK(lds <= 160 * 1024, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
+        GMC_HIP(hipFuncSetAttribute((const void*)sytrd_panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        unsigned epoch = 0;
+        for (int k = 0; k < n - 1; k += nb) {
+            const int kb = std::min(nb, n - 1 - k);
+            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.bar, p.err, n, k, kb, nb, G, epoch};
+            void* args[] = {&a};
+            GMC_HIP(hipLaunchCooperativeKernel((const void*)sytrd_panel_kernel<T>, dim3(B * G), dim3(256), args,
+                                               (unsigned)lds, s));
+            epoch += 3 * kb;
+            const int k1 = k + kb;
+            if (k1 < n) {
+                const int mt = n - k1;
+                GemmShape g{mt, mt, 2 * kb, false, true, (int64_t)n, (int64_t)n, (int64_t)n,
+                            (int64_t)n * 3 * nb, (int64_t)n * 3 * nb, (int64_t)nn, B, 0};
+                GMC_TRY(gemm<T>(s, g, T(-1), p.X + k1, p.X + (size_t)kb * n + k1, T(1), A + k1 + (size_t)k1 * n));
+            }
+        }
+    }
+    hipLaunchKernelGGL(sytrd_finish_kernel<T>, dim3((B + 255) / 256), dim3(256), 0, s, A, p.d, n, B);
+    hipLaunchKernelGGL(clear_upper_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, A, n);
+    GMC_HIP(hipGetLastError());
+
+    // ---------------- divide & conquer
+    DcBufs<T> b{p.e, w, p.Q, p.QK, p.QN, p.UB, p.dsort, p.zsort, p.dK, p.zK, p.taus, p.zh, p.EV, p.rotc, p.rots,
+                p.perm, p.lst, p.lstD, p.org, p.rank, p.rotp, p.rotj, p.kcnt, p.rcnt};
+    hipLaunchKernelGGL(dc_init_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, p.d, p.e, p.d_split, w, p.Q, n);
+    GMC_HIP(hipGetLastError());
+    for (auto& L : p.levels) {
+        DcArgs a{n, L.nmerge, B, L.d_desc};
+        hipLaunchKernelGGL(dc_prep_kernel<T>, dim3(B * L.nmerge), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_rot_kernel<T>, dim3(L.nmerge, B), dim3(256), 0, s, a, b);
+        const unsigned wg = (unsigned)(((int64_t)B * n * 64 + 255) / 256);
+        hipLaunchKernelGGL(dc_secular_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_zhat_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_vec_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_gemm_kernel<T>, dim3((unsigned)(B * L.tiles)), dim3(256), 0, s, a, b, L.tiles);
+        hipLaunchKernelGGL(dc_final1_kernel<T>, dim3(n, B), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_rank_kernel<T>, dim3((n + 255) / 256, B), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL(dc_final2_kernel<T>, dim3(n, B), dim3(256), 0, s, a, b);
+        GMC_HIP(hipGetLastError());
+    }
+
+    // ---------------- back transformation  Z <- H(0)...H(n-2) Z,  Z = Q
+    const int nbb = p.nbb;
+    T* Gm = p.Tf;
+    T* Tm = p.Tf + (size_t)B * nbb * nbb;
+    if (n > 1) {
+        const int last = ((n - 2) / nbb) * nbb;
+        for (int k = last; k >= 0; k -= nbb) {
+            const int kb = std::min(nbb, n - 1 - k);
+            const int m = n - k - 1;
+            const T* V = A + (k + 1) + (size_t)k * n;
+            T* Zs = p.Q + (k + 1);
+            GemmShape g1{kb, kb, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
+                         (int64_t)nbb * nbb, B, 0};
+            GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gm));
+            hipLaunchKernelGGL(larft_kernel<T>, dim3(B), dim3(256), 0, s, Gm, p.tau, Tm, n, k, kb, nbb);
+            GMC_HIP(hipGetLastError());
+            GemmShape g2{kb, n, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
+                         (int64_t)nn, B, 0};
+            GMC_TRY(gemm<T>(s, g2, T(1), V, Zs, T(0), p.QK));
+            GemmShape g3{kb, n, kb, false, false, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb * nbb,
+                         (int64_t)nn, (int64_t)nn, B, 0};
+            GMC_TRY(gemm<T>(s, g3, T(1), Tm, p.QK, T(0), p.QN));
+            GemmShape g4{m, n, kb, false, false, (int64_t)n, (int64_t)nbb, (int64_t)n, (int64_t)nn, (int64_t)nn,
+                         (int64_t)nn, B, 0};
+            GMC_TRY(gemm<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
+        }
+    }
+    hipLaunchKernelGGL(copy_kernel<T>, dim3(grid1((int64_t)B * nn)), dim3(256), 0, s, p.Q, A, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class T>
+int eig_check_error(hipStream_t s, EigPlan<T>& p)
+{
+    int h = 0;
+    GMC_HIP(hipMemcpyAsync(&h, p.err, sizeof(int), hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    GMC_CHECK(h == 0, "syevd: a workgroup barrier timed out (co-residency lost)");
+    return 0;
+}
+
+template struct EigPlan<double>;
+template struct EigPlan<float>;
+template int eig_plan<double>(int, int, EigPlan<double>*);
+template int eig_plan<float>(int, int, EigPlan<float>*);
+template int eig_bind<double>(EigPlan<double>*, void*);
+template int eig_bind<float>(EigPlan<float>*, void*);
+template int syevd_batched<double>(hipStream_t, EigPlan<double>&, double*, double*);
+template int syevd_batched<float>(hipStream_t, EigPlan<float>&, float*, float*);
+template int eig_check_error<double>(hipStream_t, EigPlan<double>&);
+template int eig_check_error<float>(hipStream_t, EigPlan<float>&);
+
+}  // namespace gmc

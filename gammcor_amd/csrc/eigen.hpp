@@ -1,0 +1,64 @@
+// Batched dense symmetric eigensolver for gfx950:
+//   blocked Householder tridiagonalisation (persistent panel kernel with
+//   per-problem workgroup barriers + MFMA trailing updates),
+//   divide & conquer on the tridiagonal (bottom-up merges, deflation, secular
+//   equation, Gu-Eisenstat z-hat, grouped MFMA merge GEMMs),
+//   blocked back transformation (MFMA GEMMs).
+// All B problems of one call share the order n; matrices are column-major
+// with leading dimension n and stride n*n.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace gmc {
+
+struct DcLevel {
+    int nmerge = 0;
+    int64_t tiles = 0;      // upper bound of 64x64 GEMM tiles over all merges
+    int* d_desc = nullptr;  // device: [nmerge][4] = s, m1, m2, tile offset
+    int maxm = 0;
+};
+
+template <class T>
+struct EigPlan {
+    int n = 0, batch = 0;
+    int nb = 32;            // tridiagonalisation panel width
+    int nbb = 128;          // back-transformation block width
+    int G = 1;              // workgroups cooperating on one problem
+    std::vector<DcLevel> levels;
+    unsigned char* d_split = nullptr;   // split flags (n)
+    size_t ws_bytes = 0;
+    // carved device scratch
+    T *W = nullptr, *X = nullptr, *d = nullptr, *e = nullptr, *tau = nullptr, *part = nullptr;
+    T *Q = nullptr, *QK = nullptr, *QN = nullptr, *UB = nullptr;
+    T *dsort = nullptr, *zsort = nullptr, *dK = nullptr, *zK = nullptr, *taus = nullptr, *zh = nullptr;
+    T *EV = nullptr, *rotc = nullptr, *rots = nullptr, *Tf = nullptr;
+    int *perm = nullptr, *lst = nullptr, *lstD = nullptr, *org = nullptr, *rank = nullptr;
+    int *rotp = nullptr, *rotj = nullptr, *kcnt = nullptr, *rcnt = nullptr;
+    unsigned* bar = nullptr;
+    int* err = nullptr;
+    bool bound = false;
+    EigPlan() = default;
+    EigPlan(const EigPlan&) = delete;
+    EigPlan& operator=(const EigPlan&) = delete;
+    ~EigPlan();
+};
+
+template <class T>
+int eig_plan(int n, int batch, EigPlan<T>* p);
+
+// Size the caller's scratch for eig_plan: p->ws_bytes after eig_plan.
+template <class T>
+int eig_bind(EigPlan<T>* p, void* ws);
+
+// A (batch x n x n) in: symmetric matrices (only needed fully on entry);
+// out: eigenvectors (columns), w (batch x n): eigenvalues ascending.
+template <class T>
+int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w);
+
+// Synchronises the stream and reports a barrier time-out of the panel kernel.
+template <class T>
+int eig_check_error(hipStream_t s, EigPlan<T>& p);
+
+}  // namespace gmc

@@ -1,7 +1,12 @@
 // Process-wide library state for the current HIP device (the library, like
 // the reference routines it replaces, is not re-entrant).
 #pragma once
+#include <map>
+#include <memory>
+#include <tuple>
+
 #include "common.hpp"
+#include "eigen.hpp"
 
 namespace gmc {
 
@@ -11,7 +16,13 @@ struct State {
     hipStream_t stream = nullptr;
     Workspace scratch;     // transform / solver scratch
     Workspace resident;    // integrals kept on the device across alpha points
+    std::map<std::pair<int, int>, std::unique_ptr<EigPlan<double>>> eig64;
+    std::map<std::pair<int, int>, std::unique_ptr<EigPlan<float>>> eig32;
+    template <class T>
+    int eig_plan_for(int n, int batch, EigPlan<T>** out);
     void release() {
+        eig64.clear();
+        eig32.clear();
         if (scratch.ptr) hipFree(scratch.ptr);
         if (resident.ptr) hipFree(resident.ptr);
         scratch = Workspace{};

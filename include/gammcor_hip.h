@@ -43,6 +43,12 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
                  int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
                  int64_t sc, int batch, void* stream);
 
+/* Building block exported for tests: batched symmetric eigensolver.  A holds
+ * batch column-major n x n symmetric matrices (stride n*n) and is overwritten
+ * by the eigenvectors (columns); w receives batch x n ascending eigenvalues.
+ * Replaces the DSYEVD behind Diag8 (misc.f:541-616) without its transpose. */
+int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
