@@ -1,0 +1,591 @@
+// AC-CAS on gfx950: A+-B assembly (AB_CAS, interpa.f:3659-4104), the
+// symmetric ERPA solve (ERPASYMM1, interpa.f:1652-1781) batched over alpha
+// points, and the alpha-integrand (ACEneERPA, acfd.f:4968-5043).
+//
+// A+-B depends on alpha only through factors alpha (inter-geminal terms) and
+// 1-alpha (intra-geminal one-electron shift), so every element is evaluated
+// once for alpha = 0 and alpha = 1 and A(alpha) = (1-alpha) A(0) + alpha A(1)
+// is formed on the fly by the solve.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "accas.hpp"
+#include "gemm.hpp"
+
+namespace gmc {
+
+__device__ inline double tw(const double* t, int p, int q, int r, int s) { return t[pidx(pidx(p, q), pidx(r, s))]; }
+
+struct DevSys {
+    int N, nI, nA, nO, nd;
+    const double* twono;
+    const double* occ;
+    const int* igem;
+    const double* gam;
+    const int* indn;
+};
+
+__device__ inline bool same4(const int* g, int i, int j, int k, int l)
+{
+    return g[i] == g[j] && g[j] == g[k] && g[k] == g[l];
+}
+
+// FRDM2 (interpa.f:5244-5272)
+__device__ inline double frdm2(const DevSys& s, int p, int q, int r, int s_)
+{
+    const double* occ = s.occ;
+    double v = 0.0;
+    if (p == r && q == s_ && (occ[p] == 1.0 || occ[q] == 1.0)) v += 2.0 * occ[p] * occ[q];
+    if (p == s_ && q == r && (occ[p] == 1.0 || occ[q] == 1.0)) v -= occ[p] * occ[q];
+    const int a = p - s.nI, b = q - s.nI, c = r - s.nI, d = s_ - s.nI;
+    if (a >= 0 && a < s.nA && b >= 0 && b < s.nA && c >= 0 && c < s.nA && d >= 0 && d < s.nA)
+        v = s.gam[((a * s.nA + b) * s.nA + c) * s.nA + d];
+    return v;
+}
+
+// ------------------------------------------------------------ small kernels
+__global__ void unpack_sym_kernel(const double* packed, double* M, int N)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < N) M[i + (size_t)j * N] = packed[pidx(i, j)];
+}
+
+__global__ void cicoef_kernel(const double* occ, double* c, int N)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        double v = sqrt(occ[i]);
+        c[i] = occ[i] < 0.5 ? -v : v;
+    }
+}
+
+// HNO alpha parts and AuxI/AuxIO at alpha = 0 and 1, one thread per pair p>=q.
+// h packed arrays: index pidx(p,q).
+__global__ void ab_aux_kernel(DevSys s, const double* hno, double* h0, double* h1, double* ai0, double* ai1,
+                              double* aio0, double* aio1)
+{
+    const int64_t T = (int64_t)s.N * (s.N + 1) / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T) return;
+    int p = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(p + 1) * (p + 2) / 2 <= idx) p++;
+    while ((int64_t)p * (p + 1) / 2 > idx) p--;
+    const int q = (int)(idx - (int64_t)p * (p + 1) / 2);
+    const double h = hno[p + (size_t)q * s.N];
+    const int* g = s.igem;
+    // alpha one-electron part (interpa.f:3785-3810), element (I,J) = (p,q)
+    if (g[p] != g[q]) {
+        h0[idx] = 0.0;
+        h1[idx] = h;
+    } else {
+        double aux = 0.0;
+        for (int t = 0; t < s.N; t++)
+            if (g[t] != g[p]) aux += s.occ[t] * (2.0 * tw(s.twono, t, t, p, q) - tw(s.twono, t, p, t, q));
+        h0[idx] = h + aux;
+        h1[idx] = h;
+    }
+    // AuxI, AuxIO (interpa.f:3843-3858)
+    double a0 = 0, a1 = 0, o0 = 0, o1 = 0;
+    for (int t = 0; t < s.nO; t++) {
+        const double v = s.occ[t] * (2.0 * tw(s.twono, p, q, t, t) - tw(s.twono, p, t, q, t));
+        const bool same = g[t] == g[p] && g[p] == g[q];
+        a1 += v;
+        if (same) a0 += v;
+        if (t < s.nI) {
+            o1 += v;
+            if (same) o0 += v;
+        }
+    }
+    ai0[idx] = a0;
+    ai1[idx] = a1;
+    aio0[idx] = o0;
+    aio1[idx] = o1;
+}
+
+// WMAT (interpa.f:3862-3890): one wave per (p, r), r < nO.  ld = N.
+__global__ __launch_bounds__(256) void ab_wmat_kernel(DevSys s, double* w0, double* w1)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wid >= (int64_t)s.N * s.nO) return;
+    const int p = (int)(wid % s.N), r = (int)(wid / s.N);
+    const int nO = s.nO;
+    const int tot = nO * nO * nO;
+    double a0 = 0, a1 = 0;
+    for (int x = lane; x < tot; x += 64) {
+        const int t = x / (nO * nO), w = (x / nO) % nO, u = x % nO;
+        const double v = tw(s.twono, t, w, p, u) * frdm2(s, w, u, t, r) + tw(s.twono, t, u, p, w) * frdm2(s, w, u, r, t);
+        a1 += v;
+        if (same4(s.igem, t, w, p, u)) a0 += v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+    }
+    if (lane == 0) {
+        w0[p + (size_t)r * s.N] = a0;
+        w1[p + (size_t)r * s.N] = a1;
+    }
+}
+
+// ECASSCF (interpa.f:3755-3771) with HNO before the alpha modification.
+__global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hno, double* out)
+{
+    __shared__ double red[4];
+    double e = 0.0;
+    for (int i = threadIdx.x; i < s.N; i += 256) e += 2.0 * s.occ[i] * hno[i + (size_t)i * s.N];
+    const int nO = s.nO;
+    const int64_t tot = (int64_t)nO * nO * nO * nO;
+    for (int64_t x = threadIdx.x; x < tot; x += 256) {
+        const int p = (int)(x / ((int64_t)nO * nO * nO)), q = (int)((x / (nO * nO)) % nO), r = (int)((x / nO) % nO),
+                  ss = (int)(x % nO);
+        e += frdm2(s, p, q, r, ss) * tw(s.twono, p, r, q, ss);
+    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+struct AbAux {
+    const double *h0, *h1, *ai0, *ai1, *aio0, *aio1, *w0, *w1, *c;
+};
+
+// A_{rs,pq} of interpa.f:3905-4054 at alpha = 0 (e0) and alpha = 1 (e1).
+__device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, int ir, int is, double& e0,
+                             double& e1)
+{
+    const double* occ = s.occ;
+    const int* g = s.igem;
+    const int N = s.N;
+    const int64_t iqs = pidx(is, iq), ipr = pidx(ir, ip);
+    double a0 = 0.0, a1 = 0.0;
+    if (ip == ir) {
+        a0 += (occ[ip] - occ[is]) * x.h0[iqs];
+        a1 += (occ[ip] - occ[is]) * x.h1[iqs];
+    }
+    if (is == iq) {
+        a0 += (occ[iq] - occ[ir]) * x.h0[ipr];
+        a1 += (occ[iq] - occ[ir]) * x.h1[ipr];
+    }
+    const double f0pqrs = same4(g, ip, iq, ir, is) ? 1.0 : 0.0;
+    const double xpqrs = 2.0 * tw(s.twono, ip, iq, ir, is) - tw(s.twono, ip, ir, iq, is);
+    const int nI = s.nI, nA = s.nA;
+    // T1+T2
+    if (occ[ip] * occ[ir] != 0.0) {
+        if (occ[ip] == 1.0 || occ[ir] == 1.0) {
+            const double v = occ[ip] * occ[ir] * xpqrs;
+            a0 += f0pqrs * v;
+            a1 += v;
+            if (ip == ir) {
+                a0 += occ[ip] * x.ai0[iqs];
+                a1 += occ[ip] * x.ai1[iqs];
+            }
+        } else {
+            double sum = 0.0;
+            for (int tt = 0; tt < nA; tt++)
+                for (int uu = 0; uu < nA; uu++) {
+                    const int it = nI + tt, iu = nI + uu;
+                    sum += tw(s.twono, is, iq, it, iu) * frdm2(s, ip, iu, ir, it) +
+                           tw(s.twono, is, iu, it, iq) * frdm2(s, ip, iu, it, ir);
+                }
+            if (same4(g, is, iq, nI, nI)) a0 += sum;
+            a1 += sum;
+            if (ip == ir) {
+                a0 += occ[ip] * x.aio0[iqs];
+                a1 += occ[ip] * x.aio1[iqs];
+            }
+        }
+    }
+    // T3+T4
+    if (occ[iq] * occ[is] != 0.0) {
+        if (occ[iq] == 1.0 || occ[is] == 1.0) {
+            const double v = occ[iq] * occ[is] * xpqrs;
+            a0 += f0pqrs * v;
+            a1 += v;
+            if (iq == is) {
+                a0 += occ[iq] * x.ai0[ipr];
+                a1 += occ[iq] * x.ai1[ipr];
+            }
+        } else {
+            double sum = 0.0;
+            for (int tt = 0; tt < nA; tt++)
+                for (int uu = 0; uu < nA; uu++) {
+                    const int it = nI + tt, iu = nI + uu;
+                    sum += tw(s.twono, iu, it, ip, ir) * frdm2(s, is, it, iq, iu) +
+                           tw(s.twono, iu, ir, ip, it) * frdm2(s, is, it, iu, iq);
+                }
+            if (same4(g, nI, nI, ip, ir)) a0 += sum;
+            a1 += sum;
+            if (iq == is) {
+                a0 += occ[iq] * x.aio0[ipr];
+                a1 += occ[iq] * x.aio1[ipr];
+            }
+        }
+    }
+    // T5
+    if (occ[ir] * occ[iq] != 0.0) {
+        if (occ[iq] == 1.0 || occ[ir] == 1.0) {
+            const double v = occ[iq] * occ[ir] * xpqrs;
+            a0 -= f0pqrs * v;
+            a1 -= v;
+        } else {
+            double sum = 0.0;
+            for (int tt = 0; tt < nA; tt++)
+                for (int uu = 0; uu < nA; uu++) {
+                    const int it = nI + tt, iu = nI + uu;
+                    sum += tw(s.twono, ip, it, is, iu) * frdm2(s, it, iu, iq, ir);
+                }
+            if (same4(g, ip, nI, is, nI)) a0 -= sum;
+            a1 -= sum;
+        }
+    }
+    // T6
+    if (occ[ip] * occ[is] != 0.0) {
+        if (occ[ip] == 1.0 || occ[is] == 1.0) {
+            const double v = occ[ip] * occ[is] * xpqrs;
+            a0 -= f0pqrs * v;
+            a1 -= v;
+        } else {
+            double sum = 0.0;
+            for (int tt = 0; tt < nA; tt++)
+                for (int uu = 0; uu < nA; uu++) {
+                    const int it = nI + tt, iu = nI + uu;
+                    sum += tw(s.twono, it, iq, iu, ir) * frdm2(s, is, ip, iu, it);
+                }
+            if (same4(g, nI, iq, nI, ir)) a0 -= sum;
+            a1 -= sum;
+        }
+    }
+    // WMAT(p, r) is stored for r < nO only; the reference's N x N array is
+    // zero beyond that (interpa.f:3866-3870)
+    if (is == iq && ir < s.nO) {
+        a0 -= 0.5 * x.w0[ip + (size_t)ir * N];
+        a1 -= 0.5 * x.w1[ip + (size_t)ir * N];
+    }
+    if (ip == ir && is < s.nO) {
+        a0 -= 0.5 * x.w0[iq + (size_t)is * N];
+        a1 -= 0.5 * x.w1[iq + (size_t)is * N];
+    }
+    e0 = a0;
+    e1 = a1;
+}
+
+// Main A+-B loop (interpa.f:3894-4075) and C scaling (4079-4097).
+__global__ __launch_bounds__(256) void ab_main_kernel(DevSys s, AbAux x, double* p0, double* m0, double* p1,
+                                                      double* m1)
+{
+    const int irow = blockIdx.x * blockDim.x + threadIdx.x;
+    const int icol = blockIdx.y;
+    const int nd = s.nd;
+    if (irow >= nd) return;
+    const int ir = s.indn[2 * irow] - 1, is = s.indn[2 * irow + 1] - 1;
+    const int ipp = s.indn[2 * icol] - 1, iqq = s.indn[2 * icol + 1] - 1;
+    double P0 = 0, P1 = 0, M0 = 0, M1 = 0;
+    for (int o = 0; o < 2; o++) {
+        const int ip = o == 0 ? iqq : ipp, iq = o == 0 ? ipp : iqq;
+        double e0, e1;
+        arspq(s, x, ip, iq, ir, is, e0, e1);
+        P0 += e0;
+        P1 += e1;
+        if (ir > is && ip > iq) {
+            M0 += e0;
+            M1 += e1;
+        } else if (ir > is && iq > ip) {
+            M0 -= e0;
+            M1 -= e1;
+        }
+    }
+    const double* c = x.c;
+    const double cp = (c[ipp] + c[iqq]) * (c[ir] + c[is]);
+    const double cm = (c[ipp] - c[iqq]) * (c[ir] - c[is]);
+    if (cp != 0.0) {
+        P0 = P0 / (c[ipp] + c[iqq]) / (c[ir] + c[is]);
+        P1 = P1 / (c[ipp] + c[iqq]) / (c[ir] + c[is]);
+    }
+    if (cm != 0.0) {
+        M0 = M0 / (c[ipp] - c[iqq]) / (c[ir] - c[is]);
+        M1 = M1 / (c[ipp] - c[iqq]) / (c[ir] - c[is]);
+    }
+    const size_t at = irow + (size_t)icol * nd;
+    p0[at] = P0;
+    m0[at] = M0;
+    p1[at] = P1;
+    m1[at] = M1;
+}
+
+static DevSys devsys(const AcSystem& a)
+{
+    return DevSys{a.N, a.nI, a.nA, a.nI + a.nA, a.ndimx, a.twono, a.occ, a.igem, a.gam, a.indn};
+}
+
+size_t ab_scratch_bytes(const AcSystem& s)
+{
+    const size_t N = s.N, T = N * (N + 1) / 2, nO = s.nI + s.nA;
+    return carve_bytes({N * N * 8, N * N * 8, N * N * 8, T * 8, T * 8, T * 8, T * 8, T * 8, T * 8, N * nO * 8,
+                        N * nO * 8, 8});
+}
+
+int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
+{
+    GMC_CHECK(a.N > 0 && a.ndimx > 0, "AB_CAS: empty system");
+    const int N = a.N;
+    const size_t T = (size_t)N * (N + 1) / 2, nO = a.nI + a.nA;
+    Arena ar{static_cast<char*>(scratch), ab_scratch_bytes(a)};
+    double* H = ar.take<double>((size_t)N * N);
+    double* UH = ar.take<double>((size_t)N * N);
+    double* hno = ar.take<double>((size_t)N * N);
+    double* h0 = ar.take<double>(T);
+    double* h1 = ar.take<double>(T);
+    double* ai0 = ar.take<double>(T);
+    double* ai1 = ar.take<double>(T);
+    double* aio0 = ar.take<double>(T);
+    double* aio1 = ar.take<double>(T);
+    double* w0 = ar.take<double>((size_t)N * nO);
+    double* w1 = ar.take<double>((size_t)N * nO);
+    double* d_e = ar.take<double>(1);
+    GMC_CHECK(ar.fits(), "AB_CAS: scratch layout overflow");
+    DevSys s = devsys(a);
+
+    // HNO = U h U^T (interpa.f:3702-3716)
+    hipLaunchKernelGGL(unpack_sym_kernel, dim3((N + 255) / 256, N), dim3(256), 0, st, a.xone, a.ure ? H : hno, N);
+    GMC_HIP(hipGetLastError());
+    if (a.ure) {
+        GemmShape g1{N, N, N, false, false, N, N, N, 0, 0, 0, 1, 0};
+        GMC_TRY(gemm<double>(st, g1, 1.0, a.ure, H, 0.0, UH));
+        GemmShape g2{N, N, N, false, true, N, N, N, 0, 0, 0, 1, 0};
+        GMC_TRY(gemm<double>(st, g2, 1.0, UH, a.ure, 0.0, hno));
+    }
+    hipLaunchKernelGGL(ab_ecas_kernel, dim3(1), dim3(256), 0, st, s, hno, d_e);
+    hipLaunchKernelGGL(cicoef_kernel, dim3((N + 255) / 256), dim3(256), 0, st, a.occ, out.cicoef, N);
+    hipLaunchKernelGGL(ab_aux_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, s, hno, h0, h1, ai0, ai1,
+                       aio0, aio1);
+    if (nO > 0)
+        hipLaunchKernelGGL(ab_wmat_kernel, dim3((unsigned)(((int64_t)N * nO * 64 + 255) / 256)), dim3(256), 0, st, s,
+                           w0, w1);
+    GMC_HIP(hipGetLastError());
+    AbAux x{h0, h1, ai0, ai1, aio0, aio1, w0, w1, out.cicoef};
+    GMC_CHECK(a.ndimx <= 65535, "AB_CAS: NDimX=%d exceeds the grid y limit", a.ndimx);
+    hipLaunchKernelGGL(ab_main_kernel, dim3((a.ndimx + 255) / 256, a.ndimx), dim3(256), 0, st, s, x, out.p0, out.m0,
+                       out.p1, out.m1);
+    GMC_HIP(hipGetLastError());
+    GMC_HIP(hipMemcpyAsync(&out.ecasscf, d_e, sizeof(double), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// ------------------------------------------------------------ integrand weights
+__global__ void gw_kernel(DevSys s, const double* c, double* Gw)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+    if (I >= s.nd) return;
+    const int p = s.indn[2 * I] - 1, r = s.indn[2 * I + 1] - 1;
+    const int q = s.indn[2 * J] - 1, ss = s.indn[2 * J + 1] - 1;
+    const int* g = s.igem;
+    double v = 0.0;
+    if (!(g[p] == g[r] && g[q] == g[ss] && g[p] == g[q]))
+        v = 2.0 * (c[ss] + c[q]) * (c[p] + c[r]) * tw(s.twono, p, r, q, ss);
+    Gw[I + (size_t)J * s.nd] = v;
+}
+
+__global__ __launch_bounds__(256) void wdiag_kernel(DevSys s, double* out)
+{
+    __shared__ double red[4];
+    double e = 0.0;
+    const int* g = s.igem;
+    for (int I = threadIdx.x; I < s.nd; I += 256) {
+        const int p = s.indn[2 * I] - 1, r = s.indn[2 * I + 1] - 1;
+        if (!(g[p] == g[r])) {
+            const double np = s.occ[p], nr = s.occ[r];
+            e -= (np * (1.0 - nr) + nr * (1.0 - np)) * tw(s.twono, p, r, p, r);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+int integrand_weights(hipStream_t st, const AcSystem& a, const double* cicoef, double* Gw, double* d_wdiag)
+{
+    DevSys s = devsys(a);
+    hipLaunchKernelGGL(gw_kernel, dim3((a.ndimx + 255) / 256, a.ndimx), dim3(256), 0, st, s, cicoef, Gw);
+    hipLaunchKernelGGL(wdiag_kernel, dim3(1), dim3(256), 0, st, s, d_wdiag);
+    GMC_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------ ERPA
+// A(alpha) = (1-alpha) A(0) + alpha A(1), symmetrised, into the batch buffers.
+template <class T>
+__global__ void form_kernel(const double* p0, const double* p1, const double* m0, const double* m1,
+                            const double* alphas, T* Ap, T* Am, int n)
+{
+    const int b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const double al = alphas[b];
+    const size_t ij = i + (size_t)j * n, ji = j + (size_t)i * n;
+    const double xp = 0.5 * (((1.0 - al) * p0[ij] + al * p1[ij]) + ((1.0 - al) * p0[ji] + al * p1[ji]));
+    const double xm = 0.5 * (((1.0 - al) * m0[ij] + al * m1[ij]) + ((1.0 - al) * m0[ji] + al * m1[ji]));
+    Ap[(size_t)b * n * n + ij] = (T)xp;
+    Am[(size_t)b * n * n + ij] = (T)xm;
+}
+
+// V' = V |lambda|^(1/4) column-wise; count negative eigenvalues of A+.
+template <class T>
+__global__ void sqrt_scale_kernel(T* V, const T* lam, int n, int* nneg)
+{
+    const int b = blockIdx.z, k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const T l = lam[(size_t)b * n + k];
+    const T f = sqrt(sqrt(fabs(l)));
+    if (i < n) V[(size_t)b * n * n + i + (size_t)k * n] *= f;
+    if (i == 0 && l < T(0)) atomicAdd(&nneg[b], 1);
+}
+
+template <class T>
+__global__ void symmetrize_kernel(T* M, int n)
+{
+    const int b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n || i <= j) return;
+    T* m = M + (size_t)b * n * n;
+    const T v = T(0.5) * (m[i + (size_t)j * n] + m[j + (size_t)i * n]);
+    m[i + (size_t)j * n] = v;
+    m[j + (size_t)i * n] = v;
+}
+
+// Normalisation of interpa.f:1748-1776 (2/omega Y^T A- Y = 2 omega for Y = S z)
+// and the filter of acfd.f:4976, 5013.  Yk <- Y scaled, Yf <- kept columns only.
+template <class T>
+__global__ void ycols_kernel(T* Y, T* Yf, const T* om2, double* eig, int n)
+{
+    const int b = blockIdx.z, k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double w2 = (double)om2[(size_t)b * n + k];
+    double e = w2, sc = 1.0;
+    if (w2 > 1.0e-6) {
+        e = sqrt(w2);
+        sc = 1.0 / sqrt(2.0 * e);
+    }
+    const bool keep = e > 1.0e-3 && e < 1.0e8;
+    if (i < n) {
+        const size_t at = (size_t)b * n * n + i + (size_t)k * n;
+        const T y = (T)((double)Y[at] * sc);
+        Y[at] = y;
+        Yf[at] = keep ? y : T(0);
+    }
+    if (i == 0) eig[(size_t)b * n + k] = e;
+}
+
+// W = sum_IJ Gw_IJ P_IJ with P lower-triangular valid: diag + 2 * strictly lower.
+template <class T>
+__global__ __launch_bounds__(256) void dot_lower_kernel(const double* Gw, const T* P, int n, double* partial)
+{
+    __shared__ double red[4];
+    const int b = blockIdx.y;
+    const T* pm = P + (size_t)b * n * n;
+    const int64_t total = (int64_t)n * n;
+    double acc = 0.0;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        if (i > j) acc += 2.0 * Gw[idx] * (double)pm[idx];
+        else if (i == j) acc += Gw[idx] * (double)pm[idx];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void sum_partials_kernel(const double* partial, int np, const double* wdiag, double* W, int nb)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double s = 0.0;
+    for (int i = 0; i < np; i++) s += partial[(size_t)b * np + i];
+    W[b] = s + *wdiag;
+}
+
+template <class T>
+int erpa_plan(int n, int batch, ErpaPlan<T>* p)
+{
+    p->n = n;
+    p->batch = batch;
+    const size_t nn = (size_t)n * n;
+    const size_t mat = batch * nn * sizeof(T), vec = batch * n * sizeof(T);
+    p->ws_bytes = carve_bytes({mat, mat, mat, mat, vec, vec, batch * sizeof(double),
+                               (size_t)batch * 1024 * sizeof(double)});
+    return 0;
+}
+
+template <class T>
+int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
+                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out)
+{
+    GMC_CHECK(nalpha >= 1 && nalpha <= p.batch, "ERPA: %d alpha points for a plan of %d", nalpha, p.batch);
+    GMC_CHECK(p.eig && p.eig->bound && p.eig->batch == p.batch && p.eig->n == p.n, "ERPA: eigensolver plan unbound");
+    const int n = p.n, B = nalpha;
+    const size_t nn = (size_t)n * n;
+    Arena ar{static_cast<char*>(scratch), p.ws_bytes};
+    T* E = ar.take<T>(p.batch * nn);
+    T* Mb = ar.take<T>(p.batch * nn);
+    T* Sb = ar.take<T>(p.batch * nn);
+    T* Tb = ar.take<T>(p.batch * nn);
+    T* lam = ar.take<T>(p.batch * n);
+    T* om2 = ar.take<T>(p.batch * n);
+    double* d_al = ar.take<double>(p.batch);
+    const int npart = 1024;
+    double* partial = ar.take<double>((size_t)p.batch * npart);
+    GMC_CHECK(ar.fits(), "ERPA: scratch layout overflow");
+    GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
+    GMC_HIP(hipMemsetAsync(out.nneg, 0, nalpha * sizeof(int), st));
+
+    const dim3 gsq((n + 255) / 256, n, B), blk(256);
+    // A+(alpha) -> E, A-(alpha) -> Tb
+    hipLaunchKernelGGL(form_kernel<T>, gsq, blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, E, Tb, n);
+    GMC_HIP(hipGetLastError());
+    // eig(A+): the batch plan may be larger than nalpha; pad by solving the
+    // last alpha again (cheap compared with a second plan)
+    EigPlan<T>& ep = *p.eig;
+    GMC_CHECK(ep.batch == B, "ERPA: plan batch %d != %d alpha points", ep.batch, B);
+    GMC_TRY(syevd_batched<T>(st, ep, E, lam));
+    hipLaunchKernelGGL(sqrt_scale_kernel<T>, gsq, blk, 0, st, E, lam, n, out.nneg);
+    GMC_HIP(hipGetLastError());
+    // S = V|L|^(1/2)V^T
+    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GMC_TRY(gemm<T>(st, gS, T(1), E, E, T(0), Sb));
+    // M = S A- S
+    GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GMC_TRY(gemm<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
+    GMC_TRY(gemm<T>(st, gN, T(1), Sb, Mb, T(0), Tb));
+    hipLaunchKernelGGL(symmetrize_kernel<T>, gsq, blk, 0, st, Tb, n);
+    GMC_HIP(hipGetLastError());
+    GMC_TRY(syevd_batched<T>(st, ep, Tb, om2));
+    // Y = S Z
+    GMC_TRY(gemm<T>(st, gN, T(1), Sb, Tb, T(0), Mb));
+    hipLaunchKernelGGL(ycols_kernel<T>, gsq, blk, 0, st, Mb, E, om2, out.eig, n);
+    GMC_HIP(hipGetLastError());
+    // P = Yf Yf^T (lower) and W
+    GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    GMC_TRY(gemm<T>(st, gP, T(1), E, E, T(0), Sb));
+    hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+    GMC_HIP(hipGetLastError());
+    if (Y_out) {
+        GMC_CHECK(sizeof(T) == 8, "ERPA: Y output is available from the FP64 solve");
+        GMC_HIP(hipMemcpyAsync(Y_out, Mb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    return 0;
+}
+
+template int erpa_plan<double>(int, int, ErpaPlan<double>*);
+template int erpa_plan<float>(int, int, ErpaPlan<float>*);
+template int erpa_batched<double>(hipStream_t, ErpaPlan<double>&, const AbPair&, const double*, int, const double*,
+                                  const double*, void*, ErpaOut&, double*);
+template int erpa_batched<float>(hipStream_t, ErpaPlan<float>&, const AbPair&, const double*, int, const double*,
+                                 const double*, void*, ErpaOut&, double*);
+
+}  // namespace gmc

@@ -1,0 +1,61 @@
+// AC-CAS device pipeline: A+-B assembly, ERPA solve and the alpha-integrand.
+#pragma once
+#include "common.hpp"
+#include "eigen.hpp"
+
+namespace gmc {
+
+// Static description of one system, all arrays on the device.
+struct AcSystem {
+    int N = 0, nI = 0, nA = 0, ndimx = 0;
+    const double* twono = nullptr;   // packed NO integrals (NAddr3 order), NInte2
+    const double* occ = nullptr;     // N
+    const int* igem = nullptr;       // N, geminal labels
+    const double* xone = nullptr;    // packed one-electron matrix (NO basis), N(N+1)/2
+    const double* ure = nullptr;     // N x N, URe; nullptr means identity
+    const double* gam = nullptr;     // nA^4 dense active Gamma(p,q,r,s)
+    const int* indn = nullptr;       // 2 x ndimx, IndN(1,i) at [2i], IndN(2,i) at [2i+1], 1-based
+};
+
+// A+-B at alpha = 0 and alpha = 1 (the matrices are affine in alpha),
+// column-major ndimx x ndimx, scaled by the C coefficients exactly as
+// AB_CAS (interpa.f:4079-4097).  Also ECASSCF (w/o ENuc) and CICoef.
+struct AbPair {
+    double *p0, *m0, *p1, *m1;      // device
+    double* cicoef;                 // device, N
+    double ecasscf = 0.0;
+};
+
+size_t ab_scratch_bytes(const AcSystem& s);
+int assemble_ab01(hipStream_t st, const AcSystem& s, void* scratch, AbPair& out);
+
+// Pair-integral weights of the integrand (acfd.f:4997-5032):
+//   Gw[I,J] = 2 (C_s+C_q)(C_p+C_r) (pr|qs) * [not all four in one geminal]
+// and the alpha-independent diagonal term
+//   W_diag = - sum_I [n_p(1-n_r) + n_r(1-n_p)] (pr|pr) * [not all one geminal].
+int integrand_weights(hipStream_t st, const AcSystem& s, const double* cicoef, double* Gw, double* d_wdiag);
+
+// ERPA (ERPASYMM1, interpa.f:1652-1781) for a batch of alpha points and the
+// integrand W(alpha) (ACEneERPA).  T is the arithmetic of the eigen solves.
+struct ErpaOut {
+    double* W = nullptr;      // device, nalpha integrand values
+    double* eig = nullptr;    // device, nalpha x ndimx: omega (omega^2 if <= 1e-6), ascending
+    int* nneg = nullptr;      // device, nalpha: negative eigenvalues of A+
+};
+
+template <class T>
+struct ErpaPlan {
+    int n = 0, batch = 0;
+    EigPlan<T>* eig = nullptr;   // owned by the library state
+    size_t ws_bytes = 0;         // scratch beyond the eigensolver's
+};
+
+template <class T>
+int erpa_plan(int n, int batch, ErpaPlan<T>* p);
+
+// alphas: host array of nalpha <= plan batch values.
+template <class T>
+int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
+                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out = nullptr);
+
+}  // namespace gmc

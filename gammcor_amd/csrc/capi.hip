@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "transform.hpp"
+#include "accas.hpp"
 
 namespace gmc {
 
@@ -76,6 +77,111 @@ int ensure_init()
 }  // namespace gmc
 
 using namespace gmc;
+
+// Dense active Gamma from the packed RDM2Act of AB_CAS (NAddrRDM order,
+// interpa.f:4529-4541).
+__global__ void expand_rdm2_kernel(const double* packed, double* gam, int nA)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tot = nA * nA * nA * nA;
+    if (x >= tot) return;
+    const int a = x / (nA * nA * nA), b = (x / (nA * nA)) % nA, c = (x / nA) % nA, d = x % nA;
+    const int64_t ab = (int64_t)a * nA + b + 1, cd = (int64_t)c * nA + d + 1;
+    const int64_t mx = ab > cd ? ab : cd, mn = ab > cd ? cd : ab;
+    gam[x] = packed[mx * (mx - 1) / 2 + mn - 1];
+}
+
+struct AcRun {
+    AcSystem sys;
+    AbPair ab;
+    double* Gw = nullptr;
+    double* wdiag = nullptr;
+    double* gam = nullptr;
+    char* ab_scratch = nullptr;
+};
+
+// Carve A(0), A(1), weights and the assembly scratch from st.acws.
+static int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
+                      const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
+                      AcRun& r)
+{
+    State& st = state();
+    const size_t nd2 = (size_t)ndimx * ndimx;
+    AcSystem sys;
+    sys.N = N;
+    sys.nI = nI;
+    sys.nA = nA;
+    sys.ndimx = ndimx;
+    sys.twono = twono;
+    sys.occ = occ;
+    sys.igem = igem;
+    sys.xone = xone;
+    sys.ure = ure;
+    sys.indn = indn;
+    const size_t na4 = (size_t)nA * nA * nA * nA;
+    const size_t need = carve_bytes({nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, (size_t)N * 8, (na4 + 1) * 8, 8,
+                                     ab_scratch_bytes(sys)});
+    GMC_TRY(st.acws.reserve(need));
+    Arena ar{static_cast<char*>(st.acws.ptr), need};
+    r.ab.p0 = ar.take<double>(nd2);
+    r.ab.m0 = ar.take<double>(nd2);
+    r.ab.p1 = ar.take<double>(nd2);
+    r.ab.m1 = ar.take<double>(nd2);
+    r.Gw = ar.take<double>(nd2);
+    r.ab.cicoef = ar.take<double>(N);
+    r.gam = ar.take<double>(na4 + 1);
+    r.wdiag = ar.take<double>(1);
+    r.ab_scratch = ar.take<char>(ab_scratch_bytes(sys));
+    GMC_CHECK(ar.fits(), "ACECORR: workspace layout overflow");
+    if (nA > 0) {
+        hipLaunchKernelGGL(expand_rdm2_kernel, dim3((unsigned)((na4 + 255) / 256)), dim3(256), 0, s, rdm2act, r.gam,
+                           nA);
+        GMC_HIP(hipGetLastError());
+    }
+    sys.gam = r.gam;
+    r.sys = sys;
+    return 0;
+}
+
+template <class T>
+static int ac_alpha_batches(hipStream_t s, AcRun& r, int nalpha, const double* alphas, double* walpha,
+                            double* eig_dev, int* nneg_host, double* y_dev)
+{
+    State& st = state();
+    const int nd = r.sys.ndimx;
+    const int cap = std::max(1, std::min(16, nalpha));
+    std::vector<int> nneg(nalpha, 0);
+    for (int a0 = 0; a0 < nalpha; a0 += cap) {
+        const int B = std::min(cap, nalpha - a0);
+        EigPlan<T>* ep = nullptr;
+        GMC_TRY(st.eig_plan_for<T>(nd, B, &ep));
+        GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+        GMC_TRY(eig_bind<T>(ep, st.scratch.ptr));
+        ErpaPlan<T> pl;
+        GMC_TRY(erpa_plan<T>(nd, B, &pl));
+        pl.eig = ep;
+        const size_t outb = carve_bytes({B * sizeof(double), (size_t)B * nd * sizeof(double), B * sizeof(int)});
+        GMC_TRY(st.erpaws.reserve(pl.ws_bytes + outb));
+        char* base = static_cast<char*>(st.erpaws.ptr);
+        ErpaOut out;
+        out.W = reinterpret_cast<double*>(base + pl.ws_bytes);
+        out.eig = reinterpret_cast<double*>(base + pl.ws_bytes + align_up(B * sizeof(double)));
+        out.nneg = reinterpret_cast<int*>(base + pl.ws_bytes + align_up(B * sizeof(double)) +
+                                          align_up((size_t)B * nd * sizeof(double)));
+        GMC_TRY(erpa_batched<T>(s, pl, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, out,
+                                y_dev ? y_dev + (size_t)a0 * nd * nd : nullptr));
+        GMC_TRY(eig_check_error<T>(s, *ep));
+        GMC_HIP(hipMemcpyAsync(walpha + a0, out.W, B * sizeof(double), hipMemcpyDeviceToHost, s));
+        GMC_HIP(hipMemcpyAsync(nneg.data() + a0, out.nneg, B * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (eig_dev)
+            GMC_HIP(hipMemcpyAsync(eig_dev + (size_t)a0 * nd, out.eig, (size_t)B * nd * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+        GMC_HIP(hipStreamSynchronize(s));
+    }
+    if (nneg_host)
+        for (int i = 0; i < nalpha; i++) nneg_host[i] = nneg[i];
+    return 0;
+}
 
 template <class T>
 static int syevd_dev_impl(int n, int batch, T* A, T* w, hipStream_t s)
@@ -176,6 +282,33 @@ int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream)
     hipStream_t s = stream ? (hipStream_t)stream : state().stream;
     if (prec == GMC_FP64) return syevd_dev_impl<double>(n, batch, (double*)A, (double*)w, s);
     return syevd_dev_impl<float>(n, batch, (float*)A, (float*)w, s);
+}
+
+int gmc_acecorr_dev(int prec, int nbasis, int ninact, int nact, const double* twono, const double* occ,
+                    const int* igem, const double* xone, const double* ure, const double* rdm2act, int ndimx,
+                    const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
+                    double* walpha, double* ecasscf, double* eig, int* nneg, void* stream)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR: bad sizes");
+    GMC_CHECK(prec == GMC_FP64 || prec == GMC_FP32, "ACECORR: unknown precision %d", prec);
+    hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    AcRun r;
+    GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    std::vector<double> w(nalpha);
+    if (prec == GMC_FP64)
+        GMC_TRY(ac_alpha_batches<double>(s, r, nalpha, alphas, w.data(), eig, nneg, nullptr));
+    else
+        GMC_TRY(ac_alpha_batches<float>(s, r, nalpha, alphas, w.data(), eig, nneg, nullptr));
+    double e = 0.0;
+    for (int i = 0; i < nalpha; i++) e += weights[i] * w[i];   // acfd.f:199
+    if (walpha)
+        for (int i = 0; i < nalpha; i++) walpha[i] = w[i];
+    if (ecorr) *ecorr = e;
+    if (ecasscf) *ecasscf = r.ab.ecasscf;
+    return 0;
 }
 
 }  // extern "C"

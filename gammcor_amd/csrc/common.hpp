@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <initializer_list>
 
 namespace gmc {
 
@@ -50,7 +51,8 @@ struct Workspace {
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 
-// Sub-allocator over a Workspace, used to carve per-call scratch.
+// Sub-allocator over a Workspace, used to carve per-call scratch.  Carving
+// past `cap` is recorded and must be checked with fits() before any use.
 struct Arena {
     char* base;
     size_t cap;
@@ -61,7 +63,16 @@ struct Arena {
         off = o + count * sizeof(T);
         return reinterpret_cast<T*>(base + o);
     }
+    bool fits() const { return off <= cap; }
 };
+
+// Bytes needed for a sequence of carves of the given element counts/sizes.
+inline size_t carve_bytes(std::initializer_list<size_t> bytes)
+{
+    size_t off = 0;
+    for (size_t b : bytes) off = align_up(off) + b;
+    return align_up(off);
+}
 
 inline int64_t pair_idx(int64_t i, int64_t j) {
     int64_t a = i > j ? i : j, b = i > j ? j : i;

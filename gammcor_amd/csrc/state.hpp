@@ -16,6 +16,11 @@ struct State {
     hipStream_t stream = nullptr;
     Workspace scratch;     // transform / solver scratch
     Workspace resident;    // integrals kept on the device across alpha points
+    Workspace acws;        // A(0), A(1), integrand weights, assembly scratch
+    Workspace erpaws;      // per-alpha ERPA matrices
+    const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
+    int64_t resident_n = 0;
+    double resident_probe[8] = {0};
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<double>>> eig64;
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<float>>> eig32;
     template <class T>
@@ -25,8 +30,14 @@ struct State {
         eig32.clear();
         if (scratch.ptr) hipFree(scratch.ptr);
         if (resident.ptr) hipFree(resident.ptr);
+        if (acws.ptr) hipFree(acws.ptr);
+        if (erpaws.ptr) hipFree(erpaws.ptr);
         scratch = Workspace{};
         resident = Workspace{};
+        acws = Workspace{};
+        erpaws = Workspace{};
+        resident_key = nullptr;
+        resident_n = 0;
     }
 };
 

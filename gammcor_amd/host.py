@@ -1,0 +1,141 @@
+"""Host-side mirror of the reference's AC-CAS interface over the C ABI.
+
+Names follow the reference subroutines so a caller reads the same flow:
+ACCAS pair selection (accas.f:30-99), TwoNO1 (misc.f:166), ACECORR
+(acfd.f:2), GauLeg (misc.f:828).  Arrays that the GPU consumes are torch
+tensors on ``cuda:0``; torch only provides device memory and streams.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _lib
+
+GMC_FP64, GMC_FP32 = 0, 1
+
+
+def GauLeg(x1, x2, n):
+    """Gauss-Legendre nodes and weights on [x1, x2] (misc.f:828-865): Newton
+    iteration on P_n from the Chebyshev-like initial guess."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    xm, xl = 0.5 * (x2 + x1), 0.5 * (x2 - x1)
+    for i in range(1, (n + 1) // 2 + 1):
+        z = math.cos(math.pi * (i - 0.25) / (n + 0.5))
+        for _ in range(100):
+            p1, p2 = 1.0, 0.0
+            for j in range(1, n + 1):
+                p1, p2 = ((2 * j - 1) * z * p1 - (j - 1) * p2) / j, p1
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z1, z = z, z - p1 / pp
+            if abs(z - z1) <= 3.0e-14:
+                break
+        x[i - 1], x[n - i] = xm - xl * z, xm + xl * z
+        w[i - 1] = w[n - i] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
+    return x, w
+
+
+def ACCAS_pairs(occ, nele, thr_sel_act=1e-8, thr_qvirt=1e-7, iflcore=1):
+    """IndN(2, NDimX) of the ERPA pair space (accas.f:30-99), 1-based, p > q."""
+    occ = np.asarray(occ, dtype=np.float64)
+    nb = len(occ)
+    aux = np.where(np.arange(nb) < nele, 0, 2)
+    aux[(occ < 1.0) & (occ != 0.0)] = 1
+    out = []
+    for i in range(nb):
+        for j in range(i):
+            s = aux[i] + aux[j]
+            if s in (0, 4):
+                continue
+            if aux[i] == 1 and aux[j] == 1 and abs(occ[i] - occ[j]) / occ[i] < thr_sel_act:
+                continue
+            if iflcore == 1 or (occ[i] != 1.0 and occ[j] != 1.0):
+                if abs(occ[i] + occ[j] - 2.0) > 1e-10 and abs(occ[i] + occ[j]) > thr_qvirt:
+                    out.append((i + 1, j + 1))
+    return np.array(out, dtype=np.int32).reshape(-1, 2)
+
+
+def geminals(nbasis, ninact, nact):
+    """IGem labels: 1 inactive, 2 active, 3 virtual (initia.f:1202-1216)."""
+    g = np.full(nbasis, 3, dtype=np.int32)
+    if ninact == 0:
+        g[:nact] = 1
+        g[nact:] = 2
+    else:
+        g[:ninact] = 1
+        g[ninact:ninact + nact] = 2
+    return g
+
+
+def pack_rdm2act(gamma):
+    """RDM2Act(NAddrRDM(i,j,k,l)) = Gamma(i,j,k,l) (interpa.f:4529-4541)."""
+    na = gamma.shape[0]
+    out = np.zeros(na * na * (na * na + 1) // 2)
+    for i in range(na):
+        for j in range(na):
+            ij = i * na + j + 1
+            for k in range(na):
+                for l in range(na):
+                    kl = k * na + l + 1
+                    if ij >= kl:
+                        out[ij * (ij - 1) // 2 + kl - 1] = gamma[i, j, k, l]
+    return out
+
+
+def TwoNO1(tno, ure, nbasis):
+    """In-place AO->NO transform of packed integrals (misc.f:166).  ``tno`` is
+    a host float64 array or a cuda float64 tensor, ``ure`` likewise
+    (URe(NO,AO), column-major, i.e. a Fortran-ordered numpy array)."""
+    ninte2 = len(tno)
+    if isinstance(tno, np.ndarray):
+        u = np.asfortranarray(ure, dtype=np.float64)
+        _lib.call("gmc_twono1", tno.ctypes.data, u.ctypes.data, nbasis, ninte2)
+        return tno
+    _lib.call("gmc_twono1_dev", tno.data_ptr(), ure.data_ptr(), nbasis, ninte2, None)
+    return tno
+
+
+class DeviceSystem:
+    """Device-resident inputs of one AC-CAS solve."""
+
+    def __init__(self, twono, occ, xone, gamma_act, nbasis, ninact, nact, nele, device="cuda:0", ure=None):
+        import torch
+        self.nbasis, self.ninact, self.nact = nbasis, ninact, nact
+        self.occ_host = np.asarray(occ, dtype=np.float64)
+        self.indn_host = ACCAS_pairs(self.occ_host, nele)
+        self.ndimx = len(self.indn_host)
+        dev = torch.device(device)
+        to = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+        self.twono = twono if hasattr(twono, "data_ptr") else to(twono, torch.float64)
+        self.occ = to(self.occ_host, torch.float64)
+        self.igem = to(geminals(nbasis, ninact, nact), torch.int32)
+        self.xone = to(xone, torch.float64)
+        self.rdm2act = to(pack_rdm2act(gamma_act) if nact > 0 else np.zeros(1), torch.float64)
+        self.indn = to(self.indn_host.ravel(), torch.int32)
+        self.ure = None if ure is None else to(np.asfortranarray(ure).ravel(order="F"), torch.float64)
+
+
+def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False):
+    """ECorr = sum_i w_i W(alpha_i) over all alpha points in one batched solve
+    (acfd.f:161-201).  Returns a dict with ecorr, walpha, ecasscf, nneg and,
+    if asked, the ERPA eigenvalues per alpha (device tensor)."""
+    import torch
+    al = np.ascontiguousarray(alphas, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    na = len(al)
+    ec = np.zeros(1)
+    wa = np.zeros(na)
+    es = np.zeros(1)
+    nneg = np.zeros(na, dtype=np.int32)
+    eig = torch.zeros(na, dsys.ndimx, dtype=torch.float64, device=dsys.occ.device) if return_eig else None
+    ure = dsys.ure.data_ptr() if dsys.ure is not None else None
+    _lib.call("gmc_acecorr_dev", prec, dsys.nbasis, dsys.ninact, dsys.nact, dsys.twono.data_ptr(),
+              dsys.occ.data_ptr(), dsys.igem.data_ptr(), dsys.xone.data_ptr(), ure, dsys.rdm2act.data_ptr(),
+              dsys.ndimx, dsys.indn.data_ptr(), na, al.ctypes.data, wt.ctypes.data, ec.ctypes.data, wa.ctypes.data,
+              es.ctypes.data, eig.data_ptr() if eig is not None else None, nneg.ctypes.data, None)
+    out = dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
+    if return_eig:
+        out["eig"] = eig
+    return out

@@ -43,6 +43,22 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
                  int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
                  int64_t sc, int batch, void* stream);
 
+/* ACECORR (acfd.f:2-297, alpha loop 161-201) for the CAS case with the
+ * in-core integrals: ECorr = sum_i w_i W(alpha_i), every alpha point running
+ * AB_CAS (interpa.f:3659) -> ERPASYMM1 (interpa.f:1652) -> ACEneERPA
+ * (acfd.f:4968), all alpha points batched through one solve.
+ * Device pointers: twono (NInte2 packed NO integrals), occ (N), igem (N),
+ * xone (N(N+1)/2 packed, NO basis), ure (N x N, may be NULL = identity),
+ * rdm2act (packed active 2-RDM as AB_CAS holds it, nA^2(nA^2+1)/2), indn
+ * (IndN(2,NDimX), 1-based); eig (nalpha x NDimX, may be NULL).
+ * Host pointers: alphas, weights (nalpha), outputs ecorr, walpha (nalpha),
+ * ecasscf (w/o ENuc), nneg (nalpha, negative eigenvalues of A+; may be NULL).
+ * prec: GMC_FP64 plain double path, GMC_FP32 eigen solves in float. */
+int gmc_acecorr_dev(int prec, int nbasis, int ninact, int nact, const double* twono, const double* occ,
+                    const int* igem, const double* xone, const double* ure, const double* rdm2act, int ndimx,
+                    const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
+                    double* walpha, double* ecasscf, double* eig, int* nneg, void* stream);
+
 /* Building block exported for tests: batched symmetric eigensolver.  A holds
  * batch column-major n x n symmetric matrices (stride n*n) and is overwritten
  * by the eigenvectors (columns); w receives batch x n ascending eigenvalues.
