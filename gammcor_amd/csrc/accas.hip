@@ -8,6 +8,7 @@
 // is formed on the fly by the solve.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "accas.hpp"
@@ -578,6 +579,300 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
         GMC_CHECK(sizeof(T) == 8, "ERPA: Y output is available from the FP64 solve");
         GMC_HIP(hipMemcpyAsync(Y_out, Mb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
+    return 0;
+}
+
+
+// ------------------------------------------------------------ mixed precision
+__global__ void f64_to_f32_kernel(const double* a, float* b, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+__global__ void f32_to_f64_kernel(const float* a, double* b, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+// Ogita-Aishima: lambda_i = s_ii / (1 - r_ii) with 1 - r_ii = (X^T X)_ii, and
+// the Frobenius norms for the cluster threshold delta.  S and G = X^T X are
+// valid on and below the diagonal.
+__global__ __launch_bounds__(256) void oa_stats_kernel(const double* S, const double* G, int n, double* lam,
+                                                       double* partial)
+{
+    __shared__ double red[3][4];
+    const int b = blockIdx.y;
+    const double* s = S + (size_t)b * n * n;
+    const double* g = G + (size_t)b * n * n;
+    const int64_t total = (int64_t)n * n;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        if (i > j) {
+            a0 += 2.0 * s[idx] * s[idx];
+            a1 += 2.0 * g[idx] * g[idx];
+        } else if (i == j) {
+            const double l = s[idx] / g[idx];
+            const double d = s[idx] - l;
+            a0 += d * d;
+            a1 += (1.0 - g[idx]) * (1.0 - g[idx]);
+            a2 = fmax(a2, fabs(l));
+            lam[(size_t)b * n + i] = l;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+        a2 = fmax(a2, __shfl_xor(a2, o, 64));
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = a0;
+        red[1][w] = a1;
+        red[2][w] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* pp = partial + ((size_t)b * gridDim.x + blockIdx.x) * 3;
+        pp[0] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        pp[1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        pp[2] = fmax(fmax(red[2][0], red[2][1]), fmax(red[2][2], red[2][3]));
+    }
+}
+
+__global__ void oa_delta_kernel(const double* partial, int np, double* delta, int nb)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double s0 = 0, s1 = 0, mx = 0;
+    for (int i = 0; i < np; i++) {
+        const double* pp = partial + ((size_t)b * np + i) * 3;
+        s0 += pp[0];
+        s1 += pp[1];
+        mx = fmax(mx, pp[2]);
+    }
+    delta[b] = 64.0 * 2.220446049250313e-16 * mx;
+}
+
+// E_ij = (s_ij + l_j r_ij) / (l_j - l_i), r = I - X^T X.  The cluster rule of
+// Ogita-Aishima (r_ij / 2 when |l_i - l_j| <= delta) is applied per element:
+// a pair counts as separated when its correction stays below 1/8 and its gap
+// is above the rounding level of the spectrum (delta[b] = 64 eps max|l|).
+// A close pair keeps only the orthogonalising r_ij / 2 here and is recorded
+// (lower index, upper index, Jacobi cosine and sine of its 2x2 block) so that
+// oa_pair_rot_kernel resolves it exactly after the update.
+constexpr int OA_MAXPAIRS = 4096;
+
+__global__ void oa_e_kernel(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+                            unsigned* npair, int4* pairs, double2* rot)
+{
+    const int b = blockIdx.z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    const size_t base = (size_t)b * n * n;
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const size_t at = base + hi + (size_t)lo * n;
+    const double sij = S[at];
+    const double rij = (i == j ? 1.0 : 0.0) - G[at];
+    const double li = lam[(size_t)b * n + i], lj = lam[(size_t)b * n + j];
+    const double num = sij + lj * rij, den = lj - li;
+    double e;
+    if (i != j && fabs(den) > fmax(8.0 * fabs(num), delta[b])) {
+        e = num / den;
+    } else {
+        e = 0.5 * rij;
+        if (i > j) {
+            // 2x2 block [[a, c], [c, b]] in the basis (x_lo, x_hi)
+            const double a = lam[(size_t)b * n + lo], bb = lam[(size_t)b * n + hi];
+            const double c = sij + 0.5 * (a + bb) * rij;
+            if (c != 0.0) {
+                const double tau = (bb - a) / (2.0 * c);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t);
+                const unsigned slot = atomicAdd(&npair[b], 1u);
+                if (slot < OA_MAXPAIRS) {
+                    pairs[(size_t)b * OA_MAXPAIRS + slot] = make_int4(lo, hi, 0, 0);
+                    rot[(size_t)b * OA_MAXPAIRS + slot] = make_double2(cs, t * cs);
+                }
+            }
+        }
+    }
+    E[base + i + (size_t)j * n] = e;
+}
+
+// x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
+// close pair (in record order), one workgroup per problem.
+__global__ __launch_bounds__(256) void oa_pair_rot_kernel(double* X, int n, const unsigned* npair, const int4* pairs,
+                                                          const double2* rot)
+{
+    const int b = blockIdx.x;
+    const unsigned np = min(npair[b], (unsigned)OA_MAXPAIRS);
+    double* x = X + (size_t)b * n * n;
+    for (unsigned k = 0; k < np; k++) {
+        const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
+        const double2 cr = rot[(size_t)b * OA_MAXPAIRS + k];
+        for (int r = threadIdx.x; r < n; r += 256) {
+            const double u = x[r + (size_t)pq.x * n], v = x[r + (size_t)pq.y * n];
+            x[r + (size_t)pq.x * n] = cr.x * u - cr.y * v;
+            x[r + (size_t)pq.y * n] = cr.y * u + cr.x * v;
+        }
+        __syncthreads();
+    }
+}
+
+// Eigenvalue estimates of the rotated pairs: a - t c, b + t c.
+__global__ void oa_pair_lam_kernel(double* lam, const double* S, const double* G, int n, const unsigned* npair,
+                                   const int4* pairs, const double2* rot)
+{
+    const int b = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const unsigned np = min(npair[b], (unsigned)OA_MAXPAIRS);
+    for (unsigned k = 0; k < np; k++) {
+        const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
+        const double2 cr = rot[(size_t)b * OA_MAXPAIRS + k];
+        const size_t at = (size_t)b * n * n + pq.y + (size_t)pq.x * n;
+        double* a = &lam[(size_t)b * n + pq.x];
+        double* bb = &lam[(size_t)b * n + pq.y];
+        const double c = S[at] + 0.5 * (*a + *bb) * (-G[at]);
+        const double t = cr.y / cr.x;
+        *a -= t * c;
+        *bb += t * c;
+    }
+}
+
+// One Ogita-Aishima step: X <- X + X E.  Buffers: AX, S, G, Xn (B x n x n).
+struct OaPairs {
+    unsigned* npair;    // B
+    int4* pairs;        // B x OA_MAXPAIRS
+    double2* rot;       // B x OA_MAXPAIRS
+};
+
+static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
+                     double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op)
+{
+    const size_t nn = (size_t)n * n;
+    GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GemmShape gtl{n, n, n, true, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    GMC_TRY(gemm<double>(st, gnn, 1.0, A, X, 0.0, AX));
+    GMC_TRY(gemm<double>(st, gtl, 1.0, X, AX, 0.0, S));
+    GMC_TRY(gemm<double>(st, gtl, 1.0, X, X, 0.0, G));
+    hipLaunchKernelGGL(oa_stats_kernel, dim3(npart, B), dim3(256), 0, st, S, G, n, lam, partial);
+    hipLaunchKernelGGL(oa_delta_kernel, dim3(1), dim3(64), 0, st, partial, npart, delta, B);
+    GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
+    hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
+                       op.pairs, op.rot);
+    GMC_HIP(hipGetLastError());
+    GMC_HIP(hipMemcpyAsync(Xn, X, B * nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+    GMC_TRY(gemm<double>(st, gnn, 1.0, X, AX, 1.0, Xn));
+    std::swap(X, Xn);
+    hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot);
+    hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot);
+    GMC_HIP(hipGetLastError());
+    return 0;
+}
+
+__global__ void sqrt_scale64_kernel(double* V, const double* lam, int n, int* nneg)
+{
+    const int b = blockIdx.z, k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double l = lam[(size_t)b * n + k];
+    if (i < n) V[(size_t)b * n * n + i + (size_t)k * n] *= sqrt(sqrt(fabs(l)));
+    if (i == 0 && l < 0.0) atomicAdd(&nneg[b], 1);
+}
+
+size_t erpa_mixed_bytes(int n, int batch)
+{
+    const size_t nn = (size_t)n * n, B = batch;
+    return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
+                        B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
+                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16});
+}
+
+// Mixed ERPA: eigen solves of A+ and of M = S A- S in FP32, each refined in
+// FP64 by Ogita-Aishima iterations (n_oa >= 1); S, M, Y and the integrand in FP64.
+int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
+                       const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
+                       int n_oa)
+{
+    GMC_CHECK(ep.bound && ep.batch == nalpha && ep.n == n, "ERPA mixed: eigensolver plan mismatch");
+    const int B = nalpha;
+    const size_t nn = (size_t)n * n;
+    Arena ar{static_cast<char*>(scratch), scratch_bytes};
+    double* Ap = ar.take<double>(B * nn);
+    double* Am = ar.take<double>(B * nn);
+    double* X = ar.take<double>(B * nn);
+    double* AX = ar.take<double>(B * nn);
+    double* Sb = ar.take<double>(B * nn);
+    double* Gb = ar.take<double>(B * nn);
+    double* Xn = ar.take<double>(B * nn);
+    float* E32 = ar.take<float>(B * nn);
+    float* w32 = ar.take<float>(B * n);
+    double* lam = ar.take<double>(B * n);
+    double* om2 = ar.take<double>(B * n);
+    double* d_al = ar.take<double>(B);
+    const int npart = 1024;
+    double* partial = ar.take<double>((size_t)B * npart * 3);
+    double* delta = ar.take<double>(B);
+    OaPairs op;
+    op.npair = ar.take<unsigned>(B);
+    op.pairs = ar.take<int4>((size_t)B * OA_MAXPAIRS);
+    op.rot = ar.take<double2>((size_t)B * OA_MAXPAIRS);
+    GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
+    const bool stats = getenv("GMC_OA_STATS") != nullptr;
+    auto report = [&](const char* what, int it) -> int {
+        if (!stats) return 0;
+        std::vector<unsigned> h(B);
+        GMC_HIP(hipMemcpyAsync(h.data(), op.npair, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        unsigned tot = 0;
+        for (unsigned v : h) tot += v;
+        fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems\n", what, it, tot, B);
+        return 0;
+    };
+    GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
+    GMC_HIP(hipMemsetAsync(out.nneg, 0, nalpha * sizeof(int), st));
+    const dim3 gsq((n + 255) / 256, n, B), blk(256);
+    const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
+
+    hipLaunchKernelGGL(form_kernel<double>, gsq, blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, Ap, Am, n);
+    // eig(A+) in FP32, refined in FP64
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    for (int it = 0; it < n_oa; it++) {
+        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, lam, AX, Sb, Gb, partial, npart, delta, op));
+        GMC_TRY(report("A+", it));
+    }
+    // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
+    hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
+    GMC_HIP(hipGetLastError());
+    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GMC_TRY(gemm<double>(st, gS, 1.0, X, X, 0.0, AX));
+    GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GMC_TRY(gemm<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
+    GMC_TRY(gemm<double>(st, gN, 1.0, AX, Sb, 0.0, Ap));
+    hipLaunchKernelGGL(symmetrize_kernel<double>, gsq, blk, 0, st, Ap, n);
+    // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    for (int it = 0; it < n_oa; it++) {
+        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, om2, Am, Sb, Gb, partial, npart, delta, op));
+        GMC_TRY(report("M", it));
+    }
+    // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
+    GMC_TRY(gemm<double>(st, gN, 1.0, AX, X, 0.0, Sb));
+    hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);
+    GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    GMC_TRY(gemm<double>(st, gP, 1.0, Gb, Gb, 0.0, Am));
+    hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, B), blk, 0, st, Gw, Am, n, partial);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+    GMC_HIP(hipGetLastError());
     return 0;
 }
 

@@ -58,4 +58,10 @@ template <class T>
 int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
                  const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out = nullptr);
 
+// Mixed path: FP32 eigen solves refined by n_oa Ogita-Aishima steps in FP64.
+size_t erpa_mixed_bytes(int n, int batch);
+int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
+                       const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
+                       int n_oa);
+
 }  // namespace gmc

@@ -1,6 +1,7 @@
 // extern "C" boundary of libgammcor_hip.so.  Every entry point returns 0 on
 // success and a negative status otherwise; gmc_last_error() gives the text.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -143,34 +144,56 @@ static int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const dou
     return 0;
 }
 
-template <class T>
-static int ac_alpha_batches(hipStream_t s, AcRun& r, int nalpha, const double* alphas, double* walpha,
-                            double* eig_dev, int* nneg_host, double* y_dev)
+// Run the alpha points in batches of `cap` through the ERPA route of `prec`.
+static int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, const double* alphas,
+                            double* walpha, double* eig_dev, int* nneg_host)
 {
     State& st = state();
     const int nd = r.sys.ndimx;
-    const int cap = std::max(1, std::min(16, nalpha));
+    cap = std::max(1, std::min(cap, nalpha));
     std::vector<int> nneg(nalpha, 0);
     for (int a0 = 0; a0 < nalpha; a0 += cap) {
         const int B = std::min(cap, nalpha - a0);
-        EigPlan<T>* ep = nullptr;
-        GMC_TRY(st.eig_plan_for<T>(nd, B, &ep));
-        GMC_TRY(st.scratch.reserve(ep->ws_bytes));
-        GMC_TRY(eig_bind<T>(ep, st.scratch.ptr));
-        ErpaPlan<T> pl;
-        GMC_TRY(erpa_plan<T>(nd, B, &pl));
-        pl.eig = ep;
         const size_t outb = carve_bytes({B * sizeof(double), (size_t)B * nd * sizeof(double), B * sizeof(int)});
-        GMC_TRY(st.erpaws.reserve(pl.ws_bytes + outb));
-        char* base = static_cast<char*>(st.erpaws.ptr);
         ErpaOut out;
-        out.W = reinterpret_cast<double*>(base + pl.ws_bytes);
-        out.eig = reinterpret_cast<double*>(base + pl.ws_bytes + align_up(B * sizeof(double)));
-        out.nneg = reinterpret_cast<int*>(base + pl.ws_bytes + align_up(B * sizeof(double)) +
-                                          align_up((size_t)B * nd * sizeof(double)));
-        GMC_TRY(erpa_batched<T>(s, pl, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, out,
-                                y_dev ? y_dev + (size_t)a0 * nd * nd : nullptr));
-        GMC_TRY(eig_check_error<T>(s, *ep));
+        auto bind_out = [&](size_t off) {
+            char* base = static_cast<char*>(st.erpaws.ptr) + off;
+            out.W = reinterpret_cast<double*>(base);
+            out.eig = reinterpret_cast<double*>(base + align_up(B * sizeof(double)));
+            out.nneg = reinterpret_cast<int*>(base + align_up(B * sizeof(double)) +
+                                              align_up((size_t)B * nd * sizeof(double)));
+        };
+        if (prec == GMC_FP64 || prec == GMC_FP32) {
+            auto run = [&](auto tag) -> int {
+                using T = decltype(tag);
+                EigPlan<T>* ep = nullptr;
+                GMC_TRY(st.eig_plan_for<T>(nd, B, &ep));
+                GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+                GMC_TRY(eig_bind<T>(ep, st.scratch.ptr));
+                ErpaPlan<T> pl;
+                GMC_TRY(erpa_plan<T>(nd, B, &pl));
+                pl.eig = ep;
+                GMC_TRY(st.erpaws.reserve(align_up(pl.ws_bytes) + outb));
+                bind_out(align_up(pl.ws_bytes));
+                GMC_TRY(erpa_batched<T>(s, pl, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, out, nullptr));
+                GMC_TRY(eig_check_error<T>(s, *ep));
+                return 0;
+            };
+            if (prec == GMC_FP64) GMC_TRY(run(double(0)));
+            else GMC_TRY(run(float(0)));
+        } else {
+            int n_oa = prec == GMC_MIXED2 ? 2 : 3;
+            if (const char* e = getenv("GMC_OA_STEPS")) n_oa = std::max(0, atoi(e));
+            EigPlan<float>* ep = nullptr;
+            GMC_TRY(st.eig_plan_for<float>(nd, B, &ep));
+            GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+            GMC_TRY(eig_bind<float>(ep, st.scratch.ptr));
+            const size_t mb = erpa_mixed_bytes(nd, B);
+            GMC_TRY(st.erpaws.reserve(align_up(mb) + outb));
+            bind_out(align_up(mb));
+            GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_oa));
+            GMC_TRY(eig_check_error<float>(s, *ep));
+        }
         GMC_HIP(hipMemcpyAsync(walpha + a0, out.W, B * sizeof(double), hipMemcpyDeviceToHost, s));
         GMC_HIP(hipMemcpyAsync(nneg.data() + a0, out.nneg, B * sizeof(int), hipMemcpyDeviceToHost, s));
         if (eig_dev)
@@ -195,7 +218,6 @@ static int syevd_dev_impl(int n, int batch, T* A, T* w, hipStream_t s)
     GMC_TRY(eig_check_error<T>(s, *p));
     return 0;
 }
-
 
 extern "C" {
 
@@ -284,24 +306,21 @@ int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream)
     return syevd_dev_impl<float>(n, batch, (float*)A, (float*)w, s);
 }
 
-int gmc_acecorr_dev(int prec, int nbasis, int ninact, int nact, const double* twono, const double* occ,
-                    const int* igem, const double* xone, const double* ure, const double* rdm2act, int ndimx,
-                    const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
-                    double* walpha, double* ecasscf, double* eig, int* nneg, void* stream)
+int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono,
+                    const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
+                    int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights,
+                    double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg, void* stream)
 {
     GMC_TRY(ensure_init());
     GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR: bad sizes");
-    GMC_CHECK(prec == GMC_FP64 || prec == GMC_FP32, "ACECORR: unknown precision %d", prec);
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR: unknown precision mode %d", prec);
     hipStream_t s = stream ? (hipStream_t)stream : state().stream;
     AcRun r;
     GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
     GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
     GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
     std::vector<double> w(nalpha);
-    if (prec == GMC_FP64)
-        GMC_TRY(ac_alpha_batches<double>(s, r, nalpha, alphas, w.data(), eig, nneg, nullptr));
-    else
-        GMC_TRY(ac_alpha_batches<float>(s, r, nalpha, alphas, w.data(), eig, nneg, nullptr));
+    GMC_TRY(ac_alpha_batches(s, r, prec, alpha_batch > 0 ? alpha_batch : 16, nalpha, alphas, w.data(), eig, nneg));
     double e = 0.0;
     for (int i = 0; i < nalpha; i++) e += weights[i] * w[i];   // acfd.f:199
     if (walpha)

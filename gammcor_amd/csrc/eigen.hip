@@ -35,16 +35,19 @@ __device__ inline T wave_max(T v)
     return v;
 }
 
-// Sum over the 256 threads of a workgroup; every thread gets the result.
+// Sum over the threads of a workgroup (blockDim.x / 64 waves, at most 16);
+// every thread gets the result.
 template <class T>
 __device__ T block_sum(T v, T* red)
 {
     v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
     if (lane == 0) red[wave] = v;
     __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
+    T s = 0;
+    for (int w = 0; w < nw; w++) s += red[w];
+    return s;
 }
 template <class T>
 __device__ T block_max(T v, T* red)
@@ -100,8 +103,10 @@ struct PanelArgs {
 // Rows owned by workgroup gi: 64-row blocks gi, gi+G, ...; slot q -> row.
 __device__ inline int owned_row(int q, int gi, int G) { return 64 * (gi + G * (q >> 6)) + (q & 63); }
 
+constexpr int SY_THREADS = 1024, SY_WAVES = SY_THREADS / 64;
+
 template <class T>
-__global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
+__global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 {
     extern __shared__ char smem[];
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
@@ -127,8 +132,8 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
     T* Aj = Wj + nb;
     T* a2 = Aj + nb;
     T* t12 = a2 + nb;                               // 2 nb
-    T* red = t12 + 2 * nb;                          // 8
-    T* sc = red + 8;                                // [0]=tau [1]=scal
+    T* red = t12 + 2 * nb;                          // SY_WAVES
+    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal
 
     unsigned epoch = g.epoch0;
     T tau_prev = 0;
@@ -140,7 +145,7 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
             a2[i - 1] = T(-0.5) * tau_prev * s;
         }
         __syncthreads();
-        for (int c = tid; c < i; c += 256) {
+        for (int c = tid; c < i; c += SY_THREADS) {
             T ajc = A[j + (size_t)(k + c) * n];
             Aj[c] = ajc;
             Wj[c] = W[j + (size_t)c * n] + a2[c] * ajc;
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
         __syncthreads();
         // step 1: update column j with the panel so far; norms
         T nrm = 0;
-        for (int q = tid; q < nslots; q += 256) {
+        for (int q = tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
             if (r < j || r >= n) continue;
             T a = A[r + (size_t)j * n];
@@ -190,18 +195,27 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
         }
         __syncthreads();
         const T tau = sc[0], scal = sc[1];
-        for (int c = j + 1 + tid; c < n; c += 256) v[c] = (c == j + 1) ? T(1) : scal * A[c + (size_t)j * n];
+        for (int c = j + 1 + tid; c < n; c += SY_THREADS) v[c] = (c == j + 1) ? T(1) : scal * A[c + (size_t)j * n];
         __syncthreads();
-        for (int q = wave; q < nslots; q += 4) {
+        // SYMV: one wave per owned column, four independent loads in flight
+        for (int q = wave; q < nslots; q += SY_WAVES) {
             const int r = owned_row(q, gi, G);
             if (r <= j || r >= n) continue;
             const T* col = A + (size_t)r * n;
-            T s = 0;
-            for (int c = j + 1 + lane; c < n; c += 64) s += col[c] * v[c];
-            s = wave_sum(s);
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int c = j + 1 + lane;
+            for (; c + 192 < n; c += 256) {
+                const T x0 = col[c], x1 = col[c + 64], x2 = col[c + 128], x3 = col[c + 192];
+                s0 += x0 * v[c];
+                s1 += x1 * v[c + 64];
+                s2 += x2 * v[c + 128];
+                s3 += x3 * v[c + 192];
+            }
+            for (; c < n; c += 64) s0 += col[c] * v[c];
+            const T s = wave_sum((s0 + s1) + (s2 + s3));
             if (lane == 0) y[q] = s;
         }
-        for (int c = wave; c < 2 * i; c += 4) {
+        for (int c = wave; c < 2 * i; c += SY_WAVES) {
             const int cc = c < i ? c : c - i;
             T s = 0;
             for (int q = lane; q < nslots; q += 64) {
@@ -217,14 +231,14 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
         group_barrier(bar, G, epoch, err);
 
         // step 4: w = tau (y - V t1 - W t2), w.v partial, scaled v into A
-        for (int c = tid; c < 2 * i; c += 256) {
+        for (int c = tid; c < 2 * i; c += SY_THREADS) {
             T s = 0;
             for (int q = 0; q < G; q++) s += p_dot[q * 2 * nb + c];
             t12[c] = s;
         }
         __syncthreads();
         T wv = 0;
-        for (int q = tid; q < nslots; q += 256) {
+        for (int q = tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
             if (r <= j || r >= n) continue;
             T yy = y[q];
@@ -250,7 +264,7 @@ __global__ __launch_bounds__(256) void sytrd_panel_kernel(PanelArgs<T> g)
     __syncthreads();
     // X = [V | W | V] for the trailing rows owned here
     const int k1 = k + kb;
-    for (int q = tid; q < nslots; q += 256) {
+    for (int q = tid; q < nslots; q += SY_THREADS) {
         const int r = owned_row(q, gi, G);
         if (r < k1 || r >= n) continue;
         for (int c = 0; c < kb; c++) {
@@ -287,7 +301,7 @@ __global__ void clear_upper_kernel(T* A, int n)
 
 // ====================================================================== ormtr
 // T factor (upper triangular, ld nbb) of H(k)...H(k+kb-1) = I - V T V^T from
-// the Gram matrix Gm = V^T V (ld nbb) and tau.
+// the Gram matrix Gm = V^T V (ld nbb) and tau, built in LDS (nbb <= 64).
 template <class T>
 __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int k, int kb, int nbb)
 {
@@ -295,8 +309,9 @@ __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T
     const T* G = Gm + (size_t)p * nbb * nbb;
     T* Tm = Tf + (size_t)p * nbb * nbb;
     const T* ta = tau + (size_t)p * n + k;
-    __shared__ T tmp[256];
-    for (int idx = threadIdx.x; idx < nbb * nbb; idx += 256) Tm[idx] = T(0);
+    __shared__ T t[64][65];
+    __shared__ T tmp[64];
+    for (int idx = threadIdx.x; idx < 64 * 65; idx += 256) (&t[0][0])[idx] = T(0);
     __syncthreads();
     for (int i = 0; i < kb; i++) {
         const T ti = ta[i];
@@ -304,12 +319,16 @@ __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T
         __syncthreads();
         if (threadIdx.x < i) {
             const int c = threadIdx.x;
-            T s = 0;
-            for (int c2 = c; c2 < i; c2++) s += Tm[c + (size_t)c2 * nbb] * tmp[c2];
-            Tm[c + (size_t)i * nbb] = -ti * s;
+            T acc = 0;
+            for (int c2 = c; c2 < i; c2++) acc += t[c][c2] * tmp[c2];
+            t[c][i] = -ti * acc;
         }
-        if (threadIdx.x == 0) Tm[i + (size_t)i * nbb] = ti;
+        if (threadIdx.x == 0) t[i][i] = ti;
         __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < nbb * nbb; idx += 256) {
+        const int r = idx % nbb, c = idx / nbb;
+        Tm[idx] = (r < 64 && c < 64) ? t[r][c] : T(0);
     }
 }
 
@@ -888,7 +907,7 @@ int eig_plan(int n, int batch, EigPlan<T>* p)
     GMC_HIP(hipMalloc(&p->d_split, split.size()));
     GMC_HIP(hipMemcpy(p->d_split, split.data(), split.size(), hipMemcpyHostToDevice));
 
-    p->nbb = std::min(p->nbb, n);
+    p->nbb = std::min(64, n);
     GMC_TRY(eig_layout(p, nullptr));
     return 0;
 }
@@ -921,7 +940,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     if (n > 1) {
         const int nblk = (n + 63) / 64;
         const int nslots_max = 64 * ((nblk + G - 1) / G);
-        const size_t lds = (size_t)(n + nslots_max + 5 * nb + 16) * sizeof(T);
+        const size_t lds = (size_t)(n + nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
         GMC_CHECK(lds <= 160 * 1024, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
         GMC_HIP(hipFuncSetAttribute((const void*)sytrd_panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
@@ -930,7 +949,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             const int kb = std::min(nb, n - 1 - k);
             PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.bar, p.err, n, k, kb, nb, G, epoch};
             void* args[] = {&a};
-            GMC_HIP(hipLaunchCooperativeKernel((const void*)sytrd_panel_kernel<T>, dim3(B * G), dim3(256), args,
+            GMC_HIP(hipLaunchCooperativeKernel((const void*)sytrd_panel_kernel<T>, dim3(B * G), dim3(SY_THREADS), args,
                                                (unsigned)lds, s));
             epoch += 3 * kb;
             const int k1 = k + kb;

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _lib
 
-GMC_FP64, GMC_FP32 = 0, 1
+GMC_FP64, GMC_FP32, GMC_MIXED, GMC_MIXED2 = 0, 1, 2, 3
 
 
 def GauLeg(x1, x2, n):
@@ -117,7 +117,7 @@ class DeviceSystem:
         self.ure = None if ure is None else to(np.asfortranarray(ure).ravel(order="F"), torch.float64)
 
 
-def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False):
+def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False, stream=None, alpha_batch=0):
     """ECorr = sum_i w_i W(alpha_i) over all alpha points in one batched solve
     (acfd.f:161-201).  Returns a dict with ecorr, walpha, ecasscf, nneg and,
     if asked, the ERPA eigenvalues per alpha (device tensor)."""
@@ -131,10 +131,10 @@ def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False):
     nneg = np.zeros(na, dtype=np.int32)
     eig = torch.zeros(na, dsys.ndimx, dtype=torch.float64, device=dsys.occ.device) if return_eig else None
     ure = dsys.ure.data_ptr() if dsys.ure is not None else None
-    _lib.call("gmc_acecorr_dev", prec, dsys.nbasis, dsys.ninact, dsys.nact, dsys.twono.data_ptr(),
+    _lib.call("gmc_acecorr_dev", prec, alpha_batch, dsys.nbasis, dsys.ninact, dsys.nact, dsys.twono.data_ptr(),
               dsys.occ.data_ptr(), dsys.igem.data_ptr(), dsys.xone.data_ptr(), ure, dsys.rdm2act.data_ptr(),
               dsys.ndimx, dsys.indn.data_ptr(), na, al.ctypes.data, wt.ctypes.data, ec.ctypes.data, wa.ctypes.data,
-              es.ctypes.data, eig.data_ptr() if eig is not None else None, nneg.ctypes.data, None)
+              es.ctypes.data, eig.data_ptr() if eig is not None else None, nneg.ctypes.data, stream)
     out = dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
     if return_eig:
         out["eig"] = eig

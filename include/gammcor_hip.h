@@ -20,8 +20,12 @@ extern "C" {
 #endif
 
 #define GMC_ABI_VERSION 1
-#define GMC_FP64 0
-#define GMC_FP32 1
+/* arithmetic modes of the AC-CAS solve */
+#define GMC_FP64 0    /* plain double precision throughout                     */
+#define GMC_FP32 1    /* eigen solves in float, no refinement (diagnostic)     */
+#define GMC_MIXED 2   /* float eigen solves + three Ogita-Aishima steps in double */
+#define GMC_MIXED2 3  /* float eigen solves + two Ogita-Aishima steps (faster,    */
+                      /* misses 1 uEh at NBasis=240; diagnostic)                 */
 
 const char* gmc_last_error(void);
 int gmc_version(void);
@@ -53,11 +57,13 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
  * (IndN(2,NDimX), 1-based); eig (nalpha x NDimX, may be NULL).
  * Host pointers: alphas, weights (nalpha), outputs ecorr, walpha (nalpha),
  * ecasscf (w/o ENuc), nneg (nalpha, negative eigenvalues of A+; may be NULL).
- * prec: GMC_FP64 plain double path, GMC_FP32 eigen solves in float. */
-int gmc_acecorr_dev(int prec, int nbasis, int ninact, int nact, const double* twono, const double* occ,
-                    const int* igem, const double* xone, const double* ure, const double* rdm2act, int ndimx,
-                    const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
-                    double* walpha, double* ecasscf, double* eig, int* nneg, void* stream);
+ * prec: one of the GMC_FP64 .. GMC_MIXED2 modes.  alpha_batch: alpha points
+ * solved together (0 = 16; 1 reproduces the reference's one-alpha-at-a-time
+ * ACEInteg calls). */
+int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono,
+                    const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
+                    int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights,
+                    double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg, void* stream);
 
 /* Building block exported for tests: batched symmetric eigensolver.  A holds
  * batch column-major n x n symmetric matrices (stride n*n) and is overwritten

@@ -1,0 +1,27 @@
+"""Mixed-precision AC-CAS (FP32 eigen solves + FP64 Ogita-Aishima steps)
+against the FP64 GPU path and the oracle; alpha-batch invariance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,ni,na,ne,rank,seed,nal", [(12, 2, 4, 4, 36, 0, 4), (48, 6, 4, 4, 144, 0, 8)])
+def test_mixed_vs_fp64(cuda, n, ni, na, ne, rank, seed, nal):
+    from gammcor_amd import host, synth
+    import accas_oracle as O
+    s = synth.make_system(n, ni, na, ne, rank, 0.1, seed)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    a, w = host.GauLeg(0.0, 1.0, nal)
+    d = host.DeviceSystem(tw, s.occ, s.xone, s.gamma_act, n, ni, na, s.nele)
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64, return_eig=True)
+    one = host.ACECORR(d, a, w, prec=host.GMC_FP64, alpha_batch=1)
+    assert abs(one["ecorr"] - ref["ecorr"]) < 1e-10
+    m1 = host.ACECORR(d, a, w, prec=host.GMC_MIXED, return_eig=True)
+    m2 = host.ACECORR(d, a, w, prec=host.GMC_MIXED2, return_eig=True)
+    f32 = host.ACECORR(d, a, w, prec=host.GMC_FP32)
+    e1, e2, e32 = (abs(x["ecorr"] - ref["ecorr"]) for x in (m1, m2, f32))
+    print(f"n={n}: |dE| fp32 {e32:.2e}  mixed(3 OA) {e1:.2e}  mixed(2 OA) {e2:.2e}", flush=True)
+    assert e1 < 1e-6 and e2 < 1e-6
+    de = np.max(np.abs(m1["eig"].cpu().numpy() - ref["eig"].cpu().numpy()))
+    assert de < 1e-9
