@@ -132,6 +132,8 @@ def cas_rdms(nact, nelact, seed, ref_weight=None):
     occupied (w = ref_weight), then renormalised."""
     if ref_weight is None:
         ref_weight = CI_REF_WEIGHT
+    if nact == 0:
+        return np.zeros((0, 0)), np.zeros((0, 0, 0, 0))
     na = nelact // 2
     strings = _strings(nact, na)
     ns = len(strings)
@@ -262,10 +264,11 @@ def unpack_sym(v, n):
 def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=True, **_):
     s_l, s_c, s_ci, s_h = seed * 10 + 0, seed * 10 + 1, seed * 10 + 2, seed * 10 + 3
     gamma, d = cas_rdms(nact, nelact, s_ci)
-    w, v = np.linalg.eigh(gamma)
+    w, v = np.linalg.eigh(gamma) if nact else (np.zeros(0), np.zeros((0, 0)))
     order = np.argsort(-w, kind="stable")
     w, v = w[order], v[:, order]
-    v = v * np.where(v[np.argmax(np.abs(v), axis=0), np.arange(nact)] < 0, -1.0, 1.0)[None, :]
+    if nact:
+        v = v * np.where(v[np.argmax(np.abs(v), axis=0), np.arange(nact)] < 0, -1.0, 1.0)[None, :]
     occ_act = np.abs(w) / 2.0
     d_no = np.einsum("pqrs,pa,qb,rc,sd->abcd", d, v, v, v, v, optimize=True)
     gamma2 = 0.5 * d_no
