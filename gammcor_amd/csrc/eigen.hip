@@ -94,6 +94,8 @@ __global__ void zero_u32(unsigned* p, int n)
 template <class T>
 struct PanelArgs {
     T *A, *W, *X, *d, *e, *tau, *part;
+    T* Pu;   // B x G x n: per-workgroup partial sums of the lower-triangle SYMV
+    T* vg;   // B x G x n: per-workgroup copy of v when it does not fit in LDS
     unsigned* bar;
     int* err;
     int n, k, kb, nb, G;
@@ -105,9 +107,21 @@ __device__ inline int owned_row(int q, int gi, int G) { return 64 * (gi + G * (q
 
 constexpr int SY_THREADS = 1024, SY_WAVES = SY_THREADS / 64;
 
-template <class T>
+// LDS atomic add (ds_add_f64 / ds_add_f32 on gfx950)
+__device__ inline void lds_add(double* a, double v) { __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ inline void lds_add(float* a, float v) { __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// VLDS: keep v in LDS (n <= a few thousand); otherwise in a per-workgroup
+// global copy.  The SYMV reads only the lower triangle of the trailing
+// matrix: column r gives the dot sum_{c>=r} A[c,r] v[c] for y_r and scatters
+// A[c,r] v[r] (c > r) into an LDS accumulator u, whose per-workgroup
+// partials are summed after the next group barrier.
+template <class T, bool VLDS>
 __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 {
+    // columns per wave in the SYMV: FP32 halves the HBM time per element, so
+    // it groups columns to keep the LDS atomic unit off the critical path
+    constexpr int KC = sizeof(T) == 8 ? 1 : 4;
     extern __shared__ char smem[];
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
     const int p = blockIdx.x / G, gi = blockIdx.x % G;
@@ -126,8 +140,10 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     unsigned* bar = g.bar + p * 16;
     int* err = g.err;
 
-    T* v = reinterpret_cast<T*>(smem);             // n
-    T* y = v + n;                                   // nslots
+    T* u = reinterpret_cast<T*>(smem);             // n
+    T* v = VLDS ? u + n : g.vg + ((size_t)p * G + gi) * n;
+    T* y = VLDS ? u + 2 * n : u + n;                // nslots
+    T* Pu = g.Pu + (size_t)p * G * n;
     T* Wj = y + nslots;                             // nb
     T* Aj = Wj + nb;
     T* a2 = Aj + nb;
@@ -197,24 +213,72 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         const T tau = sc[0], scal = sc[1];
         for (int c = j + 1 + tid; c < n; c += SY_THREADS) v[c] = (c == j + 1) ? T(1) : scal * A[c + (size_t)j * n];
         __syncthreads();
-        // SYMV: one wave per owned column, four independent loads in flight
-        for (int q = wave; q < nslots; q += SY_WAVES) {
-            const int r = owned_row(q, gi, G);
-            if (r <= j || r >= n) continue;
-            const T* col = A + (size_t)r * n;
-            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            int c = j + 1 + lane;
-            for (; c + 192 < n; c += 256) {
-                const T x0 = col[c], x1 = col[c + 64], x2 = col[c + 128], x3 = col[c + 192];
-                s0 += x0 * v[c];
-                s1 += x1 * v[c + 64];
-                s2 += x2 * v[c + 128];
-                s3 += x3 * v[c + 192];
+        // SYMV on the lower triangle: one wave per owned column r, four loads
+        // in flight per lane; dot part -> y, scatter part -> u (LDS atomics)
+        for (int c = j + 1 + tid; c < n; c += SY_THREADS) u[c] = T(0);
+        __syncthreads();
+        // each wave takes KC consecutive owned columns r0..r0+KC-1 (a 64-block
+        // holds 64/KC such groups): one LDS atomic per row for their scatters
+        for (int qg = wave; KC * qg < nslots; qg += SY_WAVES) {
+            const int r0 = owned_row(KC * qg, gi, G);
+            if (r0 + KC - 1 <= j || r0 >= n) continue;
+            bool ok[KC];
+            T vk[KC], s[KC];
+            const T* col[KC];
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++) {
+                const int r = r0 + kk;
+                ok[kk] = r > j && r < n;
+                vk[kk] = ok[kk] ? v[r] : T(0);
+                col[kk] = A + (size_t)(ok[kk] ? r : r0) * n;
+                s[kk] = 0;
             }
-            for (; c < n; c += 64) s0 += col[c] * v[c];
-            const T s = wave_sum((s0 + s1) + (s2 + s3));
-            if (lane == 0) y[q] = s;
+            if (KC == 1) {
+                // single column: four row chunks in flight per lane
+                T s1 = 0, s2 = 0, s3 = 0;
+                const T* c0p = col[0];
+                const T vr = vk[0];
+                int c = r0 + lane;
+                for (; c + 192 < n; c += 256) {
+                    const T x0 = c0p[c], x1 = c0p[c + 64], x2 = c0p[c + 128], x3 = c0p[c + 192];
+                    s[0] += x0 * v[c];
+                    s1 += x1 * v[c + 64];
+                    s2 += x2 * v[c + 128];
+                    s3 += x3 * v[c + 192];
+                    if (c > r0) lds_add(&u[c], x0 * vr);
+                    lds_add(&u[c + 64], x1 * vr);
+                    lds_add(&u[c + 128], x2 * vr);
+                    lds_add(&u[c + 192], x3 * vr);
+                }
+                for (; c < n; c += 64) {
+                    const T x0 = c0p[c];
+                    s[0] += x0 * v[c];
+                    if (c > r0) lds_add(&u[c], x0 * vr);
+                }
+                s[0] += (s1 + s2) + s3;
+            } else {
+                for (int c = r0 + lane; c < n; c += 64) {
+                    // v[c] is only defined for c > j: touch it only under a valid column
+                    T w = 0;
+#pragma unroll
+                    for (int kk = 0; kk < KC; kk++) {
+                        if (ok[kk] && c >= r0 + kk) {
+                            const T a = col[kk][c];
+                            s[kk] += a * v[c];
+                            if (c > r0 + kk) w += a * vk[kk];
+                        }
+                    }
+                    if (c > j && c > r0) lds_add(&u[c], w);
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++) {
+                const T t = wave_sum(s[kk]);
+                if (lane == 0 && ok[kk]) y[KC * qg + kk] = t;
+            }
         }
+        __syncthreads();
+        for (int c = j + 1 + tid; c < n; c += SY_THREADS) Pu[(size_t)gi * n + c] = u[c];
         for (int c = wave; c < 2 * i; c += SY_WAVES) {
             const int cc = c < i ? c : c - i;
             T s = 0;
@@ -242,6 +306,16 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             const int r = owned_row(q, gi, G);
             if (r <= j || r >= n) continue;
             T yy = y[q];
+            {
+                // partials of the G workgroups: eight independent loads at a time
+                T pp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                int gg = 0;
+                for (; gg + 8 <= G; gg += 8)
+#pragma unroll
+                    for (int t = 0; t < 8; t++) pp[t] += Pu[(size_t)(gg + t) * n + r];
+                for (; gg < G; gg++) pp[0] += Pu[(size_t)gg * n + r];
+                yy += ((pp[0] + pp[1]) + (pp[2] + pp[3])) + ((pp[4] + pp[5]) + (pp[6] + pp[7]));
+            }
             for (int c = 0; c < i; c++) {
                 T arc = A[r + (size_t)(k + c) * n];
                 yy -= arc * t12[c] + (W[r + (size_t)c * n] + a2[c] * arc) * t12[i + c];
@@ -834,6 +908,8 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->e, B * n * sizeof(T));
     take(p->tau, B * n * sizeof(T));
     take(p->part, B * (1 + G * (2 + 2 * nb)) * sizeof(T));
+    take(p->Pu, B * G * n * sizeof(T));
+    take(p->vg, B * G * n * sizeof(T));
     take(p->Q, B * nn * sizeof(T));
     take(p->QK, B * nn * sizeof(T));
     take(p->QN, B * nn * sizeof(T));
@@ -940,23 +1016,24 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     if (n > 1) {
         const int nblk = (n + 63) / 64;
         const int nslots_max = 64 * ((nblk + G - 1) / G);
-        const size_t lds = (size_t)(n + nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
+        const size_t small = (size_t)(nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
+        const bool vlds = (2 * (size_t)n * sizeof(T) + small) <= 160 * 1024;
+        const size_t lds = (vlds ? 2 : 1) * (size_t)n * sizeof(T) + small;
         GMC_CHECK(lds <= 160 * 1024, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
-        GMC_HIP(hipFuncSetAttribute((const void*)sytrd_panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
+        const void* kfn = vlds ? (const void*)sytrd_panel_kernel<T, true> : (const void*)sytrd_panel_kernel<T, false>;
+        GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned epoch = 0;
         for (int k = 0; k < n - 1; k += nb) {
             const int kb = std::min(nb, n - 1 - k);
-            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.bar, p.err, n, k, kb, nb, G, epoch};
+            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, n, k, kb, nb, G, epoch};
             void* args[] = {&a};
-            GMC_HIP(hipLaunchCooperativeKernel((const void*)sytrd_panel_kernel<T>, dim3(B * G), dim3(SY_THREADS), args,
-                                               (unsigned)lds, s));
+            GMC_HIP(hipLaunchCooperativeKernel(kfn, dim3(B * G), dim3(SY_THREADS), args, (unsigned)lds, s));
             epoch += 3 * kb;
             const int k1 = k + kb;
             if (k1 < n) {
                 const int mt = n - k1;
                 GemmShape g{mt, mt, 2 * kb, false, true, (int64_t)n, (int64_t)n, (int64_t)n,
-                            (int64_t)n * 3 * nb, (int64_t)n * 3 * nb, (int64_t)nn, B, 0};
+                            (int64_t)n * 3 * nb, (int64_t)n * 3 * nb, (int64_t)nn, B, 1};
                 GMC_TRY(gemm<T>(s, g, T(-1), p.X + k1, p.X + (size_t)kb * n + k1, T(1), A + k1 + (size_t)k1 * n));
             }
         }

@@ -31,6 +31,7 @@ struct EigPlan {
     size_t ws_bytes = 0;
     // carved device scratch
     T *W = nullptr, *X = nullptr, *d = nullptr, *e = nullptr, *tau = nullptr, *part = nullptr;
+    T *Pu = nullptr, *vg = nullptr;
     T *Q = nullptr, *QK = nullptr, *QN = nullptr, *UB = nullptr;
     T *dsort = nullptr, *zsort = nullptr, *dK = nullptr, *zK = nullptr, *taus = nullptr, *zh = nullptr;
     T *EV = nullptr, *rotc = nullptr, *rots = nullptr, *Tf = nullptr;
