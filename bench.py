@@ -46,11 +46,12 @@ def build_inputs(name, device):
     return s, cfg, ao
 
 
-def run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, alpha_batch=0):
+def run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, alpha_batch=0, return_eig=False):
     from gammcor_amd import host, _lib
     work.copy_(ao)
     _lib.call("gmc_twono1_dev", work.data_ptr(), ure.data_ptr(), dsys.nbasis, work.numel(), stream)
-    return host.ACECORR(dsys, alphas, weights, prec=prec, stream=stream, alpha_batch=alpha_batch)
+    return host.ACECORR(dsys, alphas, weights, prec=prec, stream=stream, alpha_batch=alpha_batch,
+                        return_eig=return_eig)
 
 
 def cpu_baseline(s, cfg, nalpha, ao_host):
@@ -102,7 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="n240_cas88")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--modes", default="")
+    ap.add_argument("--skip", default="", help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -122,25 +123,63 @@ def main():
     dsys = host.DeviceSystem(work, s.occ, s.xone, s.gamma_act, n, s.ninact, s.nact, s.nele, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
 
-    results = {}
-    modes = [("fp64_drop_in", host.GMC_FP64, 1), ("fp64_batched", host.GMC_FP64, 0),
-             ("fp32", host.GMC_FP32, 0), ("mixed", host.GMC_MIXED, 0)]
-    if args.modes:
-        modes = [m for m in modes if m[0] in args.modes.split(",")]
-    for mode, prec, ab in modes:
-        for _ in range(args.warmup):
+    # peaks of one MI355X (datasheet): HBM3E 8 TB/s; dense FP64 MFMA 78.6 TF/s,
+    # FP32 MFMA 157.3 TF/s
+    PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12}
+    KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
+               "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
+               "twono1_sandwich": ("mfma", PEAK["mfma_f64"]), "ab_cas_main": ("hbm", PEAK["hbm"])}
+
+    def timed(prec, ab, steps, warmup, profile=False):
+        for _ in range(warmup):
             run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, ab)
         torch.cuda.synchronize()
+        if profile:
+            _lib.call("gmc_prof_reset")
+            _lib.call("gmc_prof_enable", 1)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            res = run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, ab)
+        for _ in range(steps):
+            res = run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, ab, return_eig=True)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        results[mode] = dict(sec=dt / args.steps, ecorr=res["ecorr"], walpha=list(res["walpha"]))
+        dt = (time.perf_counter() - t0) / steps
+        prof = {}
+        if profile:
+            _lib.call("gmc_prof_enable", 0)
+            for name in KERNELS:
+                ms, cnt, wk = _lib.prof_get(name)
+                if cnt:
+                    prof[name] = dict(ms=ms, launches=cnt, work=wk)
+        return dict(sec=dt, ecorr=res["ecorr"], walpha=list(res["walpha"]), eig=res["eig"], prof=prof)
 
-    print(json.dumps({k: dict(sec=v["sec"], ecorr=v["ecorr"]) for k, v in results.items()}), file=sys.stderr)
-    mixed = results.get("mixed") or next(iter(results.values()))
-    fp64 = results.get("fp64_drop_in") or results.get("fp64_batched") or mixed
+    results = {"mixed": timed(host.GMC_MIXED, 0, args.steps, args.warmup, profile=True)}
+    if "fp64_batched" not in args.skip:
+        results["fp64_batched"] = timed(host.GMC_FP64, 0, 1, 1)
+    if "fp64_drop_in" not in args.skip:
+        results["fp64_drop_in"] = timed(host.GMC_FP64, 1, 1, 1)
+    mixed = results["mixed"]
+    fp64 = results.get("fp64_drop_in") or results.get("fp64_batched")
+    ref64 = results.get("fp64_batched") or fp64
+    roof = None
+    if mixed["prof"]:
+        dom = max(mixed["prof"], key=lambda k: mixed["prof"][k]["ms"])
+        d = mixed["prof"][dom]
+        bound, peak = KERNELS[dom]
+        avg_s = d["ms"] / d["launches"] * 1e-3
+        per_launch = d["work"] / d["launches"]
+        ach = per_launch / avg_s
+        roof = {"bound": bound, "kernel": dom, "achieved": ach / (1e9 if bound == "hbm" else 1e12),
+                "peak": peak / (1e9 if bound == "hbm" else 1e12), "unit": "GB/s" if bound == "hbm" else "TFLOP/s",
+                "frac": ach / peak, "traffic": None, "avg_launch_ms": d["ms"] / d["launches"],
+                "launches_per_step": d["launches"] / args.steps,
+                "share_of_step": d["ms"] / 1e3 / args.steps / mixed["sec"],
+                "work_per_launch": per_launch,
+                "work_definition": ("lower triangle of the trailing matrix read once per column, summed over the "
+                                    "panel's columns and the alpha batch" if dom.startswith("sytrd") else
+                                    "2mnk per product")}
+        roof["kernels"] = {k: dict(ms_per_step=v["ms"] / args.steps, launches=v["launches"],
+                                   achieved=v["work"] / (v["ms"] * 1e-3) / (1e9 if KERNELS[k][0] == "hbm" else 1e12))
+                           for k, v in mixed["prof"].items()}
+    deig = float((mixed["eig"] - ref64["eig"]).abs().max())
     line = {
         "metric": "AC-CAS solves/s at NBasis=240: mixed vs FP64 GPU, |dEcorr|<=1 uEh",
         "value": 1.0 / mixed["sec"],
@@ -152,17 +191,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-eig/f64",
+        "dtype": "f32 eigen solves + f64 (Ogita-Aishima refinement, transform, assembly, quadrature)",
         "data": "synthetic (seeded Cholesky-factor AO integrals, QR orbitals, random-CI RDMs)",
         "config": {"workload": f"AC-CAS {args.config}: NBasis={n}, {s.ninact} inactive, CAS({s.nelact}e,{s.nact}o), "
-                               f"NDimX={dsys.ndimx}, {nalpha} Gauss-Legendre alpha points, rank-{cfg['rank']} AO",
+                               f"NDimX={dsys.ndimx}, {nalpha} Gauss-Legendre alpha points batched, rank-{cfg['rank']} AO",
                    "nbasis": n, "ndimx": dsys.ndimx, "nalpha": nalpha},
-        "fp64_solves_per_s": 1.0 / fp64["sec"],
-        "speedup_mixed_vs_fp64": fp64["sec"] / mixed["sec"],
-        "ecorr_fp64": fp64["ecorr"],
+        "fp64_drop_in_solves_per_s": (1.0 / results["fp64_drop_in"]["sec"]) if "fp64_drop_in" in results else None,
+        "fp64_batched_solves_per_s": (1.0 / results["fp64_batched"]["sec"]) if "fp64_batched" in results else None,
+        "speedup_mixed_vs_fp64_drop_in": (results["fp64_drop_in"]["sec"] / mixed["sec"]) if "fp64_drop_in" in results
+        else None,
+        "ecorr_fp64": ref64["ecorr"],
         "ecorr_mixed": mixed["ecorr"],
-        "abs_decorr": abs(mixed["ecorr"] - fp64["ecorr"]),
-        "roofline": None,
+        "abs_decorr": abs(mixed["ecorr"] - ref64["ecorr"]),
+        "max_abs_domega": deig,
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -23,6 +23,9 @@ SIGNATURES = {
     "gmc_acecorr_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp,
                         _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_syevd_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
+    "gmc_prof_enable": [_c_int],
+    "gmc_prof_reset": [],
+    "gmc_prof_get": [ctypes.c_char_p, _vp, _vp, _vp],
     "gmc_gemm_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
                      _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
 }
@@ -60,6 +63,15 @@ def check(rc):
 
 def call(name, *args):
     return check(getattr(lib(), name)(*args))
+
+
+def prof_get(name):
+    """(total_ms, launches, total_work) of one instrumented kernel name."""
+    t = ctypes.c_double()
+    c = ctypes.c_int()
+    w = ctypes.c_double()
+    call("gmc_prof_get", name.encode(), ctypes.addressof(t), ctypes.addressof(c), ctypes.addressof(w))
+    return t.value, c.value, w.value
 
 
 def exported_symbols():

@@ -13,6 +13,7 @@
 
 #include "accas.hpp"
 #include "gemm.hpp"
+#include "prof.hpp"
 
 namespace gmc {
 
@@ -368,8 +369,12 @@ int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
     GMC_HIP(hipGetLastError());
     AbAux x{h0, h1, ai0, ai1, aio0, aio1, w0, w1, out.cicoef};
     GMC_CHECK(a.ndimx <= 65535, "AB_CAS: NDimX=%d exceeds the grid y limit", a.ndimx);
-    hipLaunchKernelGGL(ab_main_kernel, dim3((a.ndimx + 255) / 256, a.ndimx), dim3(256), 0, st, s, x, out.p0, out.m0,
-                       out.p1, out.m1);
+    {
+        // algorithmic bytes: the four ndimx^2 output matrices written once
+        ProfScope ps(st, "ab_cas_main", 4.0 * 8.0 * (double)a.ndimx * a.ndimx);
+        hipLaunchKernelGGL(ab_main_kernel, dim3((a.ndimx + 255) / 256, a.ndimx), dim3(256), 0, st, s, x, out.p0,
+                           out.m0, out.p1, out.m1);
+    }
     GMC_HIP(hipGetLastError());
     GMC_HIP(hipMemcpyAsync(&out.ecasscf, d_e, sizeof(double), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipStreamSynchronize(st));

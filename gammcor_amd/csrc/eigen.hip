@@ -5,6 +5,7 @@
 
 #include "eigen.hpp"
 #include "gemm_tile.hpp"
+#include "prof.hpp"
 
 namespace gmc {
 
@@ -1027,7 +1028,12 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             const int kb = std::min(nb, n - 1 - k);
             PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, n, k, kb, nb, G, epoch};
             void* args[] = {&a};
-            GMC_HIP(hipLaunchCooperativeKernel(kfn, dim3(B * G), dim3(SY_THREADS), args, (unsigned)lds, s));
+            double bytes = 0.0;   // lower triangle of the trailing matrix, once per column
+            for (int jj = k; jj < k + kb; jj++) bytes += 0.5 * (double)(n - jj - 1) * (double)(n - jj) * sizeof(T);
+            {
+                ProfScope ps(s, sizeof(T) == 8 ? "sytrd_panel_f64" : "sytrd_panel_f32", bytes * B);
+                GMC_HIP(hipLaunchCooperativeKernel(kfn, dim3(B * G), dim3(SY_THREADS), args, (unsigned)lds, s));
+            }
             epoch += 3 * kb;
             const int k1 = k + kb;
             if (k1 < n) {
@@ -1055,7 +1061,10 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         hipLaunchKernelGGL(dc_secular_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(dc_zhat_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(dc_vec_kernel<T>, dim3(wg), dim3(256), 0, s, a, b);
-        hipLaunchKernelGGL(dc_gemm_kernel<T>, dim3((unsigned)(B * L.tiles)), dim3(256), 0, s, a, b, L.tiles);
+        {
+            ProfScope ps(s, sizeof(T) == 8 ? "dc_merge_gemm_f64" : "dc_merge_gemm_f32", 0.0);
+            hipLaunchKernelGGL(dc_gemm_kernel<T>, dim3((unsigned)(B * L.tiles)), dim3(256), 0, s, a, b, L.tiles);
+        }
         hipLaunchKernelGGL(dc_final1_kernel<T>, dim3(n, B), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(dc_rank_kernel<T>, dim3((n + 255) / 256, B), dim3(256), 0, s, a, b);
         hipLaunchKernelGGL(dc_final2_kernel<T>, dim3(n, B), dim3(256), 0, s, a, b);

@@ -1,5 +1,6 @@
 // Batched MFMA GEMM entry points for gfx950 (tile routine in gemm_tile.hpp).
 #include "gemm_tile.hpp"
+#include "prof.hpp"
 
 namespace gmc {
 
@@ -13,6 +14,9 @@ static int launch(hipStream_t s, const GemmShape& g, T alpha, const void* A, con
         // C = beta * C  (k == 0): run the kernel with an empty k loop
     }
     constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    // algorithmic flops: 2mnk per product (half of it when only the lower triangle is produced)
+    const double flops = 2.0 * g.m * g.n * (double)g.k * g.batch * (g.uplo == 1 ? 0.5 : 1.0);
+    ProfScope ps(s, sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", flops);
     hipLaunchKernelGGL((gemm_kernel<T, BK, PTR>), grid, dim3(256), 0, s, g, alpha, A, B, beta, C);
     GMC_HIP(hipGetLastError());
     return 0;

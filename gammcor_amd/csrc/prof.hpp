@@ -1,0 +1,19 @@
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+// Disabled by default; when enabled each instrumented launch records a
+// start/stop event pair plus the algorithmic work (bytes or flops) of that
+// launch, so a caller can compute achieved bandwidth/throughput per kernel.
+#pragma once
+#include "common.hpp"
+
+namespace gmc {
+
+struct ProfScope {
+    int slot = -1;
+    hipStream_t s = nullptr;
+    ProfScope(hipStream_t s, const char* name, double work);
+    ~ProfScope();
+};
+
+bool prof_enabled();
+
+}  // namespace gmc

@@ -18,6 +18,7 @@
 // is fetched from HBM once per L2.
 #include "common.hpp"
 #include "transform.hpp"
+#include "prof.hpp"
 
 namespace gmc {
 
@@ -247,6 +248,8 @@ static int run_sandwich(hipStream_t s, const TransformPlan& p, const double* S, 
         g.col0 = c0;
         g.ncols = std::min<int64_t>(chunk, p.T - c0);
         int64_t groups = 8 * ((g.ncols + 7) / 8) * p.nbands;
+        // algorithmic flops of one half-transform column: two N^3 products (4 N^3)
+        ProfScope ps(s, "twono1_sandwich", 4.0 * (double)p.N * p.N * p.N * (double)g.ncols);
         hipLaunchKernelGGL(sandwich_kernel, dim3((unsigned)groups), dim3(256), lds, s, g);
         GMC_HIP(hipGetLastError());
     }

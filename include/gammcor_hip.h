@@ -71,6 +71,17 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
  * Replaces the DSYEVD behind Diag8 (misc.f:541-616) without its transpose. */
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
 
+/* Kernel timing (no counterpart in the reference; used by the benchmark):
+ * while enabled, instrumented launches record HIP events on their stream.
+ * gmc_prof_get sums, over the launches of one kernel name, the event time
+ * (ms), the launch count and the algorithmic work (bytes for memory-bound
+ * kernels, flops for MFMA kernels).  Names: "sytrd_panel_f64",
+ * "sytrd_panel_f32", "gemm_f64", "gemm_f32", "twono1_sandwich",
+ * "dc_merge_gemm_f64", "dc_merge_gemm_f32", "ab_cas_main". */
+int gmc_prof_enable(int on);
+int gmc_prof_reset(void);
+int gmc_prof_get(const char* name, double* total_ms, int* count, double* total_work);
+
 #ifdef __cplusplus
 }
 #endif
