@@ -39,8 +39,11 @@ int Workspace::reserve(size_t need)
 
 State& state()
 {
-    static State st;
-    return st;
+    // Never destroyed: a static destructor would call hipFree after the HIP
+    // runtime has been torn down at process exit.  gmc_finalize releases the
+    // device memory explicitly.
+    static State* st = new State;
+    return *st;
 }
 
 template <class T>

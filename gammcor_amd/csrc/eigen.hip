@@ -61,18 +61,25 @@ __device__ T block_max(T v, T* red)
     return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// Barrier among the G workgroups that cooperate on one problem.  A bounded
+// Barrier among the G workgroups that cooperate on one problem.  Each
+// workgroup publishes its epoch in a flag of its own (64 bytes apart, no
+// contended atomic); threads 0..G-1 each wait for one flag.  Release fence
+// before the publish, acquire fence after the wait (agent scope: the
+// workgroups may sit on different XCDs with separate L2 caches).  A bounded
 // spin turns a lost arrival into an error flag instead of a hang.
-__device__ inline void group_barrier(unsigned* ctr, unsigned G, unsigned& epoch, int* err)
+constexpr int BAR_STRIDE = 16;   // unsigned words between two flags
+__device__ inline void group_barrier(unsigned* flags, int gi, int G, unsigned& epoch, int* err)
 {
     __syncthreads();
     epoch += 1;
     if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(ctr, 1u);
-        const unsigned target = epoch * G;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[gi * BAR_STRIDE], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if ((int)threadIdx.x < G) {
+        const unsigned* f = &flags[threadIdx.x * BAR_STRIDE];
         long spins = 0;
-        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             if (++spins > (1L << 26)) {
                 atomicExch(err, 1);
@@ -80,7 +87,7 @@ __device__ inline void group_barrier(unsigned* ctr, unsigned G, unsigned& epoch,
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -101,10 +108,17 @@ struct PanelArgs {
     int* err;
     int n, k, kb, nb, G;
     unsigned epoch0;
+    int vec;   // 16-byte SYMV path: n a multiple of 16 / sizeof(T) and A 16-byte aligned
 };
 
 // Rows owned by workgroup gi: 64-row blocks gi, gi+G, ...; slot q -> row.
 __device__ inline int owned_row(int q, int gi, int G) { return 64 * (gi + G * (q >> 6)) + (q & 63); }
+// First slot of the first owned 64-row block that reaches row `row`.
+__device__ inline int first_slot(int row, int gi, int G)
+{
+    const int t = row - 63 - 64 * gi;
+    return t <= 0 ? 0 : 64 * ((t + 64 * G - 1) / (64 * G));
+}
 
 constexpr int SY_THREADS = 1024, SY_WAVES = SY_THREADS / 64;
 
@@ -117,12 +131,23 @@ __device__ inline void lds_add(float* a, float v) { __hip_atomic_fetch_add(a, v,
 // matrix: column r gives the dot sum_{c>=r} A[c,r] v[c] for y_r and scatters
 // A[c,r] v[r] (c > r) into an LDS accumulator u, whose per-workgroup
 // partials are summed after the next group barrier.
-template <class T, bool VLDS>
+// 16-byte aligned vector of T (global_load_dwordx4 / ds_read_b128)
+template <class T>
+struct alignas(16) Vec16 {
+    T x[16 / sizeof(T)];
+};
+template <class T>
+__device__ inline Vec16<T> ld16(const T* p) { return *reinterpret_cast<const Vec16<T>*>(p); }
+
+template <class T, bool VLDS, int KV, int UV>
 __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 {
     // columns per wave in the SYMV: FP32 halves the HBM time per element, so
     // it groups columns to keep the LDS atomic unit off the critical path
     constexpr int KC = sizeof(T) == 8 ? 1 : 4;
+    // 16-byte path (n a multiple of the vector width): KV columns per wave,
+    // UV row chunks of 64 vectors in flight per lane
+    constexpr int VN = 16 / sizeof(T);
     extern __shared__ char smem[];
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
     const int p = blockIdx.x / G, gi = blockIdx.x % G;
@@ -138,7 +163,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T* p_norm = part + 1;
     T* p_dot = part + 1 + G;
     T* p_wv = part + 1 + G + 2 * nb * G;
-    unsigned* bar = g.bar + p * 16;
+    unsigned* bar = g.bar + (size_t)p * G * BAR_STRIDE;
     int* err = g.err;
 
     T* u = reinterpret_cast<T*>(smem);             // n
@@ -156,10 +181,10 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T tau_prev = 0;
     for (int i = 0; i < kb; i++) {
         const int j = k + i;
-        if (i > 0 && tid == 0) {
-            T s = 0;
-            for (int q = 0; q < G; q++) s += p_wv[q];
-            a2[i - 1] = T(-0.5) * tau_prev * s;
+        if (i > 0) {
+            // w.v of the previous column: its G partials loaded in parallel
+            const T s = block_sum(tid < G ? p_wv[tid] : T(0), red);
+            if (tid == 0) a2[i - 1] = T(-0.5) * tau_prev * s;
         }
         __syncthreads();
         for (int c = tid; c < i; c += SY_THREADS) {
@@ -168,17 +193,36 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             Wj[c] = W[j + (size_t)c * n] + a2[c] * ajc;
         }
         __syncthreads();
-        // step 1: update column j with the panel so far; norms
+        // step 1: update column j with the panel so far; norms.  The i-term
+        // sum of a row is split over P threads and gathered in LDS (u is free
+        // here), so its loads are issued side by side.
+        const int qa = first_slot(j, gi, G), na = nslots - qa;
+        if (i > 0 && na > 0) {
+            // slot q - qa <= row < n for every row below n, so u[0..n) suffices
+            for (int q = tid; q < na; q += SY_THREADS)
+                if (owned_row(qa + q, gi, G) < n) u[q] = T(0);
+            __syncthreads();
+            const int P = max(1, min(i, SY_THREADS / na));
+            for (int idx = tid; idx < na * P; idx += SY_THREADS) {
+                const int qq = idx % na, part = idx / na;
+                const int r = owned_row(qa + qq, gi, G);
+                if (r < j || r >= n) continue;
+                T acc = 0;
+                for (int c = part; c < i; c += P) {
+                    const T arc = A[r + (size_t)(k + c) * n];
+                    acc += arc * Wj[c] + (W[r + (size_t)c * n] + a2[c] * arc) * Aj[c];
+                }
+                lds_add(&u[qq], acc);
+            }
+            __syncthreads();
+        }
         T nrm = 0;
-        for (int q = tid; q < nslots; q += SY_THREADS) {
+        for (int q = qa + tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
             if (r < j || r >= n) continue;
             T a = A[r + (size_t)j * n];
             if (i > 0) {
-                for (int c = 0; c < i; c++) {
-                    T arc = A[r + (size_t)(k + c) * n];
-                    a -= arc * Wj[c] + (W[r + (size_t)c * n] + a2[c] * arc) * Aj[c];
-                }
+                a -= u[q - qa];
                 A[r + (size_t)j * n] = a;
             }
             if (r == j) g.d[(size_t)p * n + j] = a;
@@ -187,12 +231,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         }
         nrm = block_sum(nrm, red);
         if (tid == 0) p_norm[gi] = nrm;
-        group_barrier(bar, G, epoch, err);
+        group_barrier(bar, gi, G, epoch, err);
 
         // step 2: reflector, v into LDS; step 3: SYMV and panel dots
+        const T xn2 = block_sum(tid < G ? p_norm[tid] : T(0), red);
         if (tid == 0) {
-            T alpha = *p_alpha, xn2 = 0;
-            for (int q = 0; q < G; q++) xn2 += p_norm[q];
+            const T alpha = *p_alpha;
             T tau, beta, scal;
             if (xn2 == T(0)) {
                 tau = 0;
@@ -218,6 +262,100 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         // in flight per lane; dot part -> y, scatter part -> u (LDS atomics)
         for (int c = j + 1 + tid; c < n; c += SY_THREADS) u[c] = T(0);
         __syncthreads();
+        if (g.vec) {
+            // 16-byte path: a wave takes KV consecutive owned columns; lane l
+            // reads rows c..c+VN-1 (c = chunk + VN*l) of each, UV chunks in
+            // flight.  The first chunk (diagonal block) is masked, the rest
+            // are not: every row there lies below all KV columns.
+            for (int qg = wave; KV * qg < nslots; qg += SY_WAVES) {
+                const int r0 = owned_row(KV * qg, gi, G);
+                if (r0 + KV - 1 <= j || r0 >= n) continue;
+                T vk[KV], s[KV];
+                const T* col[KV];
+#pragma unroll
+                for (int kk = 0; kk < KV; kk++) {
+                    const int r = r0 + kk;
+                    const bool okk = r > j && r < n;
+                    vk[kk] = okk ? v[r] : T(0);
+                    col[kk] = A + (size_t)(okk ? r : r0) * n;
+                    s[kk] = 0;
+                }
+                const int base = r0 & ~63;
+                {
+                    const int c = base + VN * lane;
+                    if (c < n) {
+                        const Vec16<T> vv = ld16(v + c);
+                        T w[VN];
+#pragma unroll
+                        for (int e = 0; e < VN; e++) w[e] = 0;
+#pragma unroll
+                        for (int kk = 0; kk < KV; kk++) {
+                            const int r = r0 + kk;
+                            if (r > j && r < n) {
+                                const Vec16<T> a = ld16(col[kk] + c);
+#pragma unroll
+                                for (int e = 0; e < VN; e++) {
+                                    const int row = c + e;
+                                    if (row >= r) s[kk] += a.x[e] * vv.x[e];
+                                    if (row > r) w[e] += a.x[e] * vk[kk];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int e = 0; e < VN; e++)
+                            if (c + e > j && c + e > r0) lds_add(&u[c + e], w[e]);
+                    }
+                }
+                int c = base + 64 * VN + VN * lane;
+                for (; c + (UV - 1) * 64 * VN < n; c += UV * 64 * VN) {
+                    Vec16<T> a[UV][KV], vv[UV];
+#pragma unroll
+                    for (int uu = 0; uu < UV; uu++) {
+#pragma unroll
+                        for (int kk = 0; kk < KV; kk++) a[uu][kk] = ld16(col[kk] + c + uu * 64 * VN);
+                        vv[uu] = ld16(v + c + uu * 64 * VN);
+                    }
+#pragma unroll
+                    for (int uu = 0; uu < UV; uu++) {
+                        T w[VN];
+#pragma unroll
+                        for (int e = 0; e < VN; e++) w[e] = 0;
+#pragma unroll
+                        for (int kk = 0; kk < KV; kk++)
+#pragma unroll
+                            for (int e = 0; e < VN; e++) {
+                                s[kk] += a[uu][kk].x[e] * vv[uu].x[e];
+                                w[e] += a[uu][kk].x[e] * vk[kk];
+                            }
+#pragma unroll
+                        for (int e = 0; e < VN; e++) lds_add(&u[c + uu * 64 * VN + e], w[e]);
+                    }
+                }
+                for (; c < n; c += 64 * VN) {
+                    const Vec16<T> vv = ld16(v + c);
+                    T w[VN];
+#pragma unroll
+                    for (int e = 0; e < VN; e++) w[e] = 0;
+#pragma unroll
+                    for (int kk = 0; kk < KV; kk++) {
+                        const Vec16<T> a = ld16(col[kk] + c);
+#pragma unroll
+                        for (int e = 0; e < VN; e++) {
+                            s[kk] += a.x[e] * vv.x[e];
+                            w[e] += a.x[e] * vk[kk];
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < VN; e++) lds_add(&u[c + e], w[e]);
+                }
+#pragma unroll
+                for (int kk = 0; kk < KV; kk++) {
+                    const T t = wave_sum(s[kk]);
+                    const int r = r0 + kk;
+                    if (lane == 0 && r > j && r < n) y[KV * qg + kk] = t;
+                }
+            }
+        } else
         // each wave takes KC consecutive owned columns r0..r0+KC-1 (a 64-block
         // holds 64/KC such groups): one LDS atomic per row for their scatters
         for (int qg = wave; KC * qg < nslots; qg += SY_WAVES) {
@@ -293,35 +431,51 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             s = wave_sum(s);
             if (lane == 0) p_dot[gi * 2 * nb + c] = s;
         }
-        group_barrier(bar, G, epoch, err);
+        group_barrier(bar, gi, G, epoch, err);
 
-        // step 4: w = tau (y - V t1 - W t2), w.v partial, scaled v into A
-        for (int c = tid; c < 2 * i; c += SY_THREADS) {
-            T s = 0;
-            for (int q = 0; q < G; q++) s += p_dot[q * 2 * nb + c];
-            t12[c] = s;
+        // step 4: w = tau (y - V t1 - W t2), w.v partial, scaled v into A.
+        // t12[c] = sum over the G partials, split over PP threads per c.
+        if (i > 0) {
+            const int nc = 2 * i;
+            for (int c = tid; c < nc; c += SY_THREADS) t12[c] = T(0);
+            __syncthreads();
+            const int PP = max(1, min(G, SY_THREADS / nc));
+            for (int idx = tid; idx < nc * PP; idx += SY_THREADS) {
+                const int c = idx % nc, part = idx / nc;
+                T acc = 0;
+                for (int q = part; q < G; q += PP) acc += p_dot[q * 2 * nb + c];
+                lds_add(&t12[c], acc);
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        // y[q] += (the G workgroups' scatter partials) - (panel terms), each
+        // row's G + i terms split over P threads
+        {
+            const int qb = first_slot(j + 1, gi, G), nbq = nslots - qb, nterm = G + i;
+            const int P = nbq > 0 ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
+            for (int idx = tid; idx < nbq * P; idx += SY_THREADS) {
+                const int q = qb + idx % nbq, part = idx / nbq;
+                const int r = owned_row(q, gi, G);
+                if (r <= j || r >= n) continue;
+                T acc = 0;
+                for (int t = part; t < nterm; t += P) {
+                    if (t < G) {
+                        acc += Pu[(size_t)t * n + r];
+                    } else {
+                        const int c = t - G;
+                        const T arc = A[r + (size_t)(k + c) * n];
+                        acc -= arc * t12[c] + (W[r + (size_t)c * n] + a2[c] * arc) * t12[i + c];
+                    }
+                }
+                lds_add(&y[q], acc);
+            }
+            __syncthreads();
+        }
         T wv = 0;
         for (int q = tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
             if (r <= j || r >= n) continue;
-            T yy = y[q];
-            {
-                // partials of the G workgroups: eight independent loads at a time
-                T pp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                int gg = 0;
-                for (; gg + 8 <= G; gg += 8)
-#pragma unroll
-                    for (int t = 0; t < 8; t++) pp[t] += Pu[(size_t)(gg + t) * n + r];
-                for (; gg < G; gg++) pp[0] += Pu[(size_t)gg * n + r];
-                yy += ((pp[0] + pp[1]) + (pp[2] + pp[3])) + ((pp[4] + pp[5]) + (pp[6] + pp[7]));
-            }
-            for (int c = 0; c < i; c++) {
-                T arc = A[r + (size_t)(k + c) * n];
-                yy -= arc * t12[c] + (W[r + (size_t)c * n] + a2[c] * arc) * t12[i + c];
-            }
-            T wr = tau * yy;
+            T wr = tau * y[q];
             W[r + (size_t)i * n] = wr;
             wv += wr * v[r];
             A[r + (size_t)j * n] = v[r];
@@ -329,12 +483,11 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         wv = block_sum(wv, red);
         if (tid == 0) p_wv[gi] = wv;
         tau_prev = tau;
-        group_barrier(bar, G, epoch, err);
+        group_barrier(bar, gi, G, epoch, err);
     }
-    if (tid == 0) {
-        T s = 0;
-        for (int q = 0; q < G; q++) s += p_wv[q];
-        a2[kb - 1] = T(-0.5) * tau_prev * s;
+    {
+        const T s = block_sum(tid < G ? p_wv[tid] : T(0), red);
+        if (tid == 0) a2[kb - 1] = T(-0.5) * tau_prev * s;
     }
     __syncthreads();
     // X = [V | W | V] for the trailing rows owned here
@@ -920,7 +1073,7 @@ static int eig_layout(EigPlan<T>* p, char* base)
     for (int** v : {&p->perm, &p->lst, &p->lstD, &p->org, &p->rank, &p->rotp, &p->rotj, &p->kcnt, &p->rcnt})
         take(*v, B * n * sizeof(int));
     take(p->Tf, B * 2 * nbb * nbb * sizeof(T));
-    take(p->bar, (B + 1) * 16 * sizeof(unsigned));
+    take(p->bar, (B * G + 1) * BAR_STRIDE * sizeof(unsigned));
     take(p->err, 64);
     p->ws_bytes = align_up(off);
     return 0;
@@ -1009,7 +1162,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     GMC_CHECK(p.bound, "syevd: plan not bound to a workspace");
     const int n = p.n, B = p.batch, nb = p.nb, G = p.G;
     const size_t nn = (size_t)n * n;
-    hipLaunchKernelGGL(zero_u32, dim3((B * 16 + 16 + 255) / 256), dim3(256), 0, s, p.bar, B * 16 + 16);
+    hipLaunchKernelGGL(zero_u32, dim3(((B * G + 1) * BAR_STRIDE + 255) / 256), dim3(256), 0, s, p.bar,
+                       (B * G + 1) * BAR_STRIDE);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, (unsigned*)p.err, 16);
     GMC_HIP(hipGetLastError());
 
@@ -1021,12 +1175,19 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         const bool vlds = (2 * (size_t)n * sizeof(T) + small) <= 160 * 1024;
         const size_t lds = (vlds ? 2 : 1) * (size_t)n * sizeof(T) + small;
         GMC_CHECK(lds <= 160 * 1024, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
-        const void* kfn = vlds ? (const void*)sytrd_panel_kernel<T, true> : (const void*)sytrd_panel_kernel<T, false>;
+        // 16-byte SYMV shape per type (measured at n=2664/4524, batch 16):
+        // FP32 8 columns x 1 chunk, FP64 4 columns x 2 chunks in flight
+        constexpr int KV = sizeof(T) == 8 ? 4 : 8, UV = sizeof(T) == 8 ? 2 : 1;
+        const void* kfn = vlds ? (const void*)sytrd_panel_kernel<T, true, KV, UV>
+                               : (const void*)sytrd_panel_kernel<T, false, KV, UV>;
         GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned epoch = 0;
         for (int k = 0; k < n - 1; k += nb) {
             const int kb = std::min(nb, n - 1 - k);
-            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, n, k, kb, nb, G, epoch};
+            const int vec = (n % (16 / (int)sizeof(T)) == 0) && (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                            (reinterpret_cast<uintptr_t>(p.vg) % 16 == 0);
+            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, n, k, kb, nb, G, epoch,
+                           vec};
             void* args[] = {&a};
             double bytes = 0.0;   // lower triangle of the trailing matrix, once per column
             for (int jj = k; jj < k + kb; jj++) bytes += 0.5 * (double)(n - jj - 1) * (double)(n - jj) * sizeof(T);

@@ -1,0 +1,141 @@
+"""Per-kernel summaries of rocprofv3 CSV output for this library's kernels.
+
+    python -m gammcor_amd.profsum stats <dir>            # kernel_stats.csv / kernel_trace.csv
+    python -m gammcor_amd.profsum pmc <dir> [--calib-first 3 --calib-mib 1024]
+                                                        # counter_collection.csv, KiB per dispatch
+    python -m gammcor_amd.profsum copy [--mib M] [--reps R]
+                                                        # device copy of known size (calibration run)
+
+`<dir>` is the output directory of a `rocprofv3 -f csv -d <dir> -o run ...`
+run (the CSV files are found below it).  For `pmc`, a run that starts with
+plain device copies of known size (`bench.py --copy-calibration-mib 1024`)
+calibrates the counter: the known bytes divided by the copy kernel's reading
+give the factor that turns the counter into bytes for the other kernels.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import os
+import sys
+
+
+def _find(d, pattern):
+    hits = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not hits:
+        raise SystemExit(f"no {pattern} under {d}")
+    return hits[0]
+
+
+def _col(row, *names):
+    for n in names:
+        if n in row:
+            return row[n]
+    raise KeyError(f"none of {names} in {list(row)}")
+
+
+def kernel_stats(d):
+    """[(name, calls, total_ms, avg_us, min_us, max_us)] sorted by total time."""
+    try:
+        path = _find(d, "*kernel_stats.csv")
+        out = []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                out.append((_col(r, "Name", "KernelName"), int(_col(r, "Calls")),
+                            float(_col(r, "TotalDurationNs")) * 1e-6, float(_col(r, "AverageNs")) * 1e-3,
+                            float(_col(r, "MinNs")) * 1e-3, float(_col(r, "MaxNs")) * 1e-3))
+        return path, sorted(out, key=lambda x: -x[2])
+    except SystemExit:
+        path = _find(d, "*kernel_trace.csv")
+        agg = collections.defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                dur = (int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))) * 1e-3
+                agg[_col(r, "Kernel_Name")].append(dur)
+        out = [(k, len(v), sum(v) * 1e-3, sum(v) / len(v), min(v), max(v)) for k, v in agg.items()]
+        return path, sorted(out, key=lambda x: -x[2])
+
+
+def pmc_per_kernel(d, calib_first=0):
+    """{counter: {kernel: [values per dispatch]}} from counter_collection.csv,
+    and {counter: [values]} of the first `calib_first` dispatches of a copy
+    kernel (the calibration copies run before anything else of that name)."""
+    path = _find(d, "*counter_collection.csv")
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(_col(r, "Dispatch_Id")), _col(r, "Counter_Name"), _col(r, "Kernel_Name"),
+                         float(_col(r, "Counter_Value"))))
+    rows.sort(key=lambda x: x[0])
+    res = collections.defaultdict(lambda: collections.defaultdict(list))
+    cal = collections.defaultdict(list)
+    for _, ctr, k, v in rows:
+        if len(cal[ctr]) < calib_first and "copy" in k.lower():
+            cal[ctr].append(v)
+            continue
+        res[ctr][k].append(v)
+    return path, res, cal
+
+
+def copy_run(mib, reps):
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copy: {reps} x {mib} MiB read + {mib} MiB written (float32 copy_)", flush=True)
+
+
+def _short(name, width=90):
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("dir")
+    a.add_argument("--top", type=int, default=25)
+    b = sub.add_parser("pmc")
+    b.add_argument("dir")
+    b.add_argument("--calib-first", type=int, default=0,
+                   help="the first N copy-kernel dispatches are calibration copies of --calib-mib MiB")
+    b.add_argument("--calib-mib", type=int, default=1024)
+    b.add_argument("--top", type=int, default=12)
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=1024)
+    c.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args(argv)
+
+    if args.cmd == "copy":
+        copy_run(args.mib, args.reps)
+        return
+    if args.cmd == "stats":
+        path, rows = kernel_stats(args.dir)
+        tot = sum(r[2] for r in rows)
+        print(f"# source: {os.path.relpath(path, args.dir)}; total kernel time {tot:.1f} ms")
+        print(f"{'total_ms':>10} {'share':>6} {'calls':>6} {'avg_us':>10} {'min_us':>10} {'max_us':>10}  kernel")
+        for name, n, t, avg, mn, mx in rows[: args.top]:
+            print(f"{t:10.1f} {100 * t / tot:5.1f}% {n:6d} {avg:10.1f} {mn:10.1f} {mx:10.1f}  {_short(name)}")
+        return
+    path, res, cal = pmc_per_kernel(args.dir, args.calib_first)
+    factor = {}
+    for ctr, vals in cal.items():
+        if vals and min(vals) > 0:
+            factor[ctr] = args.calib_mib * 1024.0 / (sum(vals) / len(vals))   # true KiB per counted KiB
+            print(f"# calibration {ctr}: {len(vals)} copies of {args.calib_mib} MiB counted {sum(vals) / len(vals):.1f} KiB")
+    for ctr, ks in res.items():
+        f = factor.get(ctr)
+        print(f"# {ctr} (KiB per dispatch{'' if f is None else f', calibration factor {f:.4f} from a {args.calib_mib} MiB copy'})")
+        rows = sorted(((k, sum(v) / len(v), len(v)) for k, v in ks.items()), key=lambda x: -x[1] * x[2])
+        for k, mean, n in rows[: args.top]:
+            extra = "" if f is None else f"  calibrated {mean * f / 1024.0:12.1f} MiB"
+            print(f"{mean:14.1f} KiB x {n:5d}{extra}  {_short(k)}")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -36,7 +36,8 @@ def _mats(kind, n, batch, seed):
 @pytest.mark.parametrize("kind,n,batch", [("rand", 1, 1), ("rand", 2, 2), ("rand", 5, 3), ("rand", 33, 2),
                                           ("rand", 100, 1), ("rand", 257, 4), ("rand", 600, 2),
                                           ("clustered", 130, 2), ("degenerate", 97, 2), ("diag", 70, 1),
-                                          ("graded", 120, 1)])
+                                          ("graded", 120, 1), ("rand", 256, 3), ("rand", 452, 2),
+                                          ("rand", 1024, 1), ("clustered", 516, 2)])
 def test_syevd(cuda, prec, kind, n, batch):
     import torch
     from gammcor_amd import _lib
