@@ -26,6 +26,15 @@ SIGNATURES = {
     "gmc_prof_enable": [_c_int],
     "gmc_prof_reset": [],
     "gmc_prof_get": [ctypes.c_char_p, _vp, _vp, _vp],
+    "gmc_ab_cas": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                   _c_double, _vp],
+    "gmc_erpasymm1": [_vp, _vp, _vp, _vp, _c_int, _vp],
+    "gmc_aceneerpa": [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int],
+    "gmc_aceinteg": [_c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int,
+                     _c_double, _vp, _vp, _vp, _vp],
+    "gmc_acecorr": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
+                    _vp, _vp, _vp, _vp, _vp, _vp],
+    "gmc_forget_resident": [],
     "gmc_gemm_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
                      _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
 }

@@ -529,7 +529,8 @@ int erpa_plan(int n, int batch, ErpaPlan<T>* p)
 
 template <class T>
 int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
-                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out)
+                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out, double* S_out,
+                 double* Am_out)
 {
     GMC_CHECK(nalpha >= 1 && nalpha <= p.batch, "ERPA: %d alpha points for a plan of %d", nalpha, p.batch);
     GMC_CHECK(p.eig && p.eig->bound && p.eig->batch == p.batch && p.eig->n == p.n, "ERPA: eigensolver plan unbound");
@@ -553,8 +554,10 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     // A+(alpha) -> E, A-(alpha) -> Tb
     hipLaunchKernelGGL(form_kernel<T>, gsq, blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, E, Tb, n);
     GMC_HIP(hipGetLastError());
-    // eig(A+): the batch plan may be larger than nalpha; pad by solving the
-    // last alpha again (cheap compared with a second plan)
+    if (Am_out) {
+        GMC_CHECK(sizeof(T) == 8, "ERPA: A- output is available from the FP64 solve");
+        GMC_HIP(hipMemcpyAsync(Am_out, Tb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
     EigPlan<T>& ep = *p.eig;
     GMC_CHECK(ep.batch == B, "ERPA: plan batch %d != %d alpha points", ep.batch, B);
     GMC_TRY(syevd_batched<T>(st, ep, E, lam));
@@ -563,6 +566,10 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     // S = V|L|^(1/2)V^T
     GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm<T>(st, gS, T(1), E, E, T(0), Sb));
+    if (S_out) {
+        GMC_CHECK(sizeof(T) == 8, "ERPA: A+^(1/2) output is available from the FP64 solve");
+        GMC_HIP(hipMemcpyAsync(S_out, Sb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
     // M = S A- S
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
@@ -574,12 +581,14 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     GMC_TRY(gemm<T>(st, gN, T(1), Sb, Tb, T(0), Mb));
     hipLaunchKernelGGL(ycols_kernel<T>, gsq, blk, 0, st, Mb, E, om2, out.eig, n);
     GMC_HIP(hipGetLastError());
-    // P = Yf Yf^T (lower) and W
-    GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-    GMC_TRY(gemm<T>(st, gP, T(1), E, E, T(0), Sb));
-    hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
-    GMC_HIP(hipGetLastError());
+    // P = Yf Yf^T (lower) and W (skipped without integrand weights)
+    if (Gw) {
+        GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+        GMC_TRY(gemm<T>(st, gP, T(1), E, E, T(0), Sb));
+        hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+        GMC_HIP(hipGetLastError());
+    }
     if (Y_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: Y output is available from the FP64 solve");
         GMC_HIP(hipMemcpyAsync(Y_out, Mb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -884,8 +893,41 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
 template int erpa_plan<double>(int, int, ErpaPlan<double>*);
 template int erpa_plan<float>(int, int, ErpaPlan<float>*);
 template int erpa_batched<double>(hipStream_t, ErpaPlan<double>&, const AbPair&, const double*, int, const double*,
-                                  const double*, void*, ErpaOut&, double*);
+                                  const double*, void*, ErpaOut&, double*, double*, double*);
 template int erpa_batched<float>(hipStream_t, ErpaPlan<float>&, const AbPair&, const double*, int, const double*,
-                                 const double*, void*, ErpaOut&, double*);
+                                 const double*, void*, ErpaOut&, double*, double*, double*);
+
+// ------------------------------------------------------------ integrand from given Y
+// Columns of Y whose Eig lies in (1e-3, 1e8) (acfd.f:4976, 5013), others zero.
+__global__ void filter_cols_kernel(const double* Y, const double* eig, double* Yf, int n)
+{
+    const int k = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double e = eig[k];
+    const size_t at = i + (size_t)k * n;
+    Yf[at] = (e > 1.0e-3 && e < 1.0e8) ? Y[at] : 0.0;
+}
+
+size_t integrand_from_y_bytes(int n) { return carve_bytes({(size_t)n * n * 8, (size_t)n * n * 8, 1024 * 8}); }
+
+int integrand_from_y(hipStream_t st, int n, const double* Y, const double* eig, const double* Gw, const double* wdiag,
+                     void* scratch, double* W)
+{
+    const size_t nn = (size_t)n * n;
+    Arena ar{static_cast<char*>(scratch), integrand_from_y_bytes(n)};
+    double* Yf = ar.take<double>(nn);
+    double* P = ar.take<double>(nn);
+    const int npart = 1024;
+    double* partial = ar.take<double>(npart);
+    GMC_CHECK(ar.fits(), "ACEneERPA: scratch layout overflow");
+    hipLaunchKernelGGL(filter_cols_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, Y, eig, Yf, n);
+    GemmShape gP{n, n, n, false, true, n, n, n, 0, 0, 0, 1, 1};
+    GMC_TRY(gemm<double>(st, gP, 1.0, Yf, Yf, 0.0, P));
+    hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, 1), dim3(256), 0, st, Gw, P, n, partial);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, W, 1);
+    GMC_HIP(hipGetLastError());
+    return 0;
+}
 
 }  // namespace gmc

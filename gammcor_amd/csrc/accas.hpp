@@ -54,9 +54,18 @@ template <class T>
 int erpa_plan(int n, int batch, ErpaPlan<T>* p);
 
 // alphas: host array of nalpha <= plan batch values.
+// Optional outputs (FP64 only): Y_out the normalised Y = A+^(1/2) Z columns,
+// S_out = A+^(1/2), Am_out the symmetrised A-.  Gw == nullptr skips the integrand.
 template <class T>
 int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
-                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out = nullptr);
+                 const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out = nullptr,
+                 double* S_out = nullptr, double* Am_out = nullptr);
+
+// W = sum Gw o (Yf Yf^T) + wdiag for a given normalised Y and Eig (ACEneERPA,
+// acfd.f:4968-5043), Yf the columns with 1e-3 < Eig < 1e8.
+size_t integrand_from_y_bytes(int n);
+int integrand_from_y(hipStream_t st, int n, const double* Y, const double* eig, const double* Gw, const double* wdiag,
+                     void* scratch, double* W);
 
 // Mixed path: FP32 eigen solves refined by n_oa Ogita-Aishima steps in FP64.
 size_t erpa_mixed_bytes(int n, int batch);

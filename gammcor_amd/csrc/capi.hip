@@ -7,6 +7,7 @@
 
 #include "../../include/gammcor_hip.h"
 #include "state.hpp"
+#include "acrun.hpp"
 #include "common.hpp"
 #include "gemm.hpp"
 #include "transform.hpp"
@@ -95,17 +96,10 @@ __global__ void expand_rdm2_kernel(const double* packed, double* gam, int nA)
     gam[x] = packed[mx * (mx - 1) / 2 + mn - 1];
 }
 
-struct AcRun {
-    AcSystem sys;
-    AbPair ab;
-    double* Gw = nullptr;
-    double* wdiag = nullptr;
-    double* gam = nullptr;
-    char* ab_scratch = nullptr;
-};
+namespace gmc {
 
 // Carve A(0), A(1), weights and the assembly scratch from st.acws.
-static int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
+int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
                       const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
                       AcRun& r)
 {
@@ -148,7 +142,7 @@ static int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const dou
 }
 
 // Run the alpha points in batches of `cap` through the ERPA route of `prec`.
-static int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, const double* alphas,
+int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, const double* alphas,
                             double* walpha, double* eig_dev, int* nneg_host)
 {
     State& st = state();
@@ -209,6 +203,8 @@ static int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalp
     return 0;
 }
 
+}  // namespace gmc
+
 template <class T>
 static int syevd_dev_impl(int n, int batch, T* A, T* w, hipStream_t s)
 {
@@ -256,36 +252,6 @@ int gmc_twono1_dev(double* tno, const double* ure, int nbasis, int64_t ninte2, v
     return 0;
 }
 
-int gmc_twono1(double* tno, const double* ure, int nbasis, int64_t ninte2)
-{
-    GMC_TRY(ensure_init());
-    State& st = state();
-    const size_t bytes = (size_t)ninte2 * sizeof(double);
-    double* d_tno = nullptr;
-    double* d_u = nullptr;
-    GMC_HIP(hipMalloc(&d_tno, bytes));
-    GMC_HIP(hipMalloc(&d_u, (size_t)nbasis * nbasis * sizeof(double)));
-    int rc = 0;
-    if (hipMemcpyAsync(d_tno, tno, bytes, hipMemcpyHostToDevice, st.stream) != hipSuccess ||
-        hipMemcpyAsync(d_u, ure, (size_t)nbasis * nbasis * sizeof(double), hipMemcpyHostToDevice, st.stream) !=
-            hipSuccess) {
-        set_error("TwoNO1: host-to-device copy failed");
-        rc = -1;
-    }
-    if (rc == 0) rc = gmc_twono1_dev(d_tno, d_u, nbasis, ninte2, st.stream);
-    if (rc == 0 && hipMemcpyAsync(tno, d_tno, bytes, hipMemcpyDeviceToHost, st.stream) != hipSuccess) {
-        set_error("TwoNO1: device-to-host copy failed");
-        rc = -1;
-    }
-    if (hipStreamSynchronize(st.stream) != hipSuccess && rc == 0) {
-        set_error("TwoNO1: stream synchronize failed");
-        rc = -1;
-    }
-    hipFree(d_tno);
-    hipFree(d_u);
-    return rc;
-}
-
 int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, const void* A, int64_t lda,
                  int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc, int64_t sc,
                  int batch, void* stream)
@@ -318,6 +284,7 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
     GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR: bad sizes");
     GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR: unknown precision mode %d", prec);
     hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    state().sys_valid = false;   // acws is overwritten: drop the host-call system cache
     AcRun r;
     GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
     GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));

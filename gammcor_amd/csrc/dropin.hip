@@ -1,0 +1,381 @@
+// Host-pointer entry points behind the reference's own subroutine interfaces
+// (TwoNO1, AB_CAS, ERPASYMM1, ACEneERPA, ACEInteg, ACECORR).  The packed
+// integrals stay resident on the device across calls (keyed by the host
+// address, the size and a sample of values), and so do A(0), A(1) and the
+// integrand weights of the last system (keyed by a hash of its small arrays),
+// so ACECORR's one-alpha-at-a-time ACEInteg calls upload and assemble once.
+#include <cstring>
+#include <vector>
+
+#include "../../include/gammcor_hip.h"
+#include "accas.hpp"
+#include "acrun.hpp"
+#include "common.hpp"
+#include "eigen.hpp"
+#include "state.hpp"
+#include "transform.hpp"
+
+namespace gmc {
+
+namespace {
+
+constexpr int NPROBE = 64;
+
+void probe_values(const double* h, int64_t n, double* out)
+{
+    for (int k = 0; k < NPROBE; k++) out[k] = h[(n - 1) * k / (NPROBE - 1)];
+}
+
+uint64_t fnv(uint64_t h, const void* p, size_t bytes)
+{
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < bytes; i++) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// Small per-system arrays staged on the device.
+struct Staged {
+    double *occ, *xone, *ure, *rdm2act, *cicoef;
+    int *igem, *indn;
+};
+
+// The cached system (valid while state().sys_valid): A(0), A(1), Gw, wdiag
+// live in st.acws, the staged small arrays in st.sysio.
+AcRun g_run;
+Staged g_staged{};
+
+}  // namespace
+
+// Device copy of the packed integrals `host` (NInte2 doubles); uploads only
+// when the address, size or sampled values differ from the resident copy.
+static int resident_twono(hipStream_t s, const double* host, int64_t ninte2, const double** dev)
+{
+    State& st = state();
+    double probe[NPROBE];
+    probe_values(host, ninte2, probe);
+    if (st.resident_key == host && st.resident_n == ninte2 &&
+        std::memcmp(probe, st.resident_probe, sizeof(probe)) == 0) {
+        *dev = st.resident.as<double>();
+        return 0;
+    }
+    st.resident_key = nullptr;
+    st.sys_valid = false;
+    GMC_TRY(st.resident.reserve((size_t)ninte2 * sizeof(double)));
+    GMC_HIP(hipMemcpyAsync(st.resident.ptr, host, (size_t)ninte2 * sizeof(double), hipMemcpyHostToDevice, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    st.resident_key = host;
+    st.resident_n = ninte2;
+    std::memcpy(st.resident_probe, probe, sizeof(probe));
+    *dev = st.resident.as<double>();
+    return 0;
+}
+
+// Stage the small arrays of one system in `ws`.
+static int stage_small(hipStream_t s, Workspace& ws, int N, int nA, int ndimx, const double* occ, const double* xone,
+                       const double* ure, const double* rdm2act, const int* igem, const int* indn, Staged& d)
+{
+    const size_t na2 = (size_t)nA * nA, nrdm = na2 * (na2 + 1) / 2 + 1, ninte1 = (size_t)N * (N + 1) / 2;
+    const size_t bytes = carve_bytes({(size_t)N * 8, ninte1 * 8, (size_t)N * N * 8, nrdm * 8, (size_t)N * 8,
+                                      (size_t)N * 4, (size_t)2 * ndimx * 4});
+    GMC_TRY(ws.reserve(bytes));
+    Arena ar{static_cast<char*>(ws.ptr), bytes};
+    d.occ = ar.take<double>(N);
+    d.xone = ar.take<double>(ninte1);
+    d.ure = ar.take<double>((size_t)N * N);
+    d.rdm2act = ar.take<double>(nrdm);
+    d.cicoef = ar.take<double>(N);
+    d.igem = ar.take<int>(N);
+    d.indn = ar.take<int>((size_t)2 * ndimx);
+    GMC_CHECK(ar.fits(), "drop-in: staging layout overflow");
+    auto up = [&](void* dst, const void* src, size_t b) -> int {
+        if (src && b) GMC_HIP(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
+        return 0;
+    };
+    GMC_TRY(up(d.occ, occ, (size_t)N * 8));
+    GMC_TRY(up(d.xone, xone, ninte1 * 8));
+    GMC_TRY(up(d.ure, ure, (size_t)N * N * 8));
+    GMC_TRY(up(d.rdm2act, rdm2act, (nrdm - 1) * 8));
+    GMC_TRY(up(d.igem, igem, (size_t)N * 4));
+    GMC_TRY(up(d.indn, indn, (size_t)2 * ndimx * 4));
+    return 0;
+}
+
+// Prepared system for host inputs: reuse the cached A(0), A(1), Gw when the
+// integrals are the resident ones and the small arrays hash the same.
+static int host_system(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, int64_t ninte2,
+                       const double* occ, const int* igem, const double* xone, const double* ure,
+                       const double* rdm2act, const int* indn, AcRun** out)
+{
+    State& st = state();
+    GMC_CHECK(N >= 2 && ndimx >= 1 && nI >= 0 && nA >= 0 && nI + nA <= N, "drop-in: bad sizes N=%d nI=%d nA=%d", N,
+              nI, nA);
+    GMC_CHECK(ninte2 == (int64_t)N * (N + 1) / 2 * ((int64_t)N * (N + 1) / 2 + 1) / 2,
+              "drop-in: NInte2=%lld does not match NBasis=%d", (long long)ninte2, N);
+    const double* d_tw = nullptr;
+    GMC_TRY(resident_twono(s, twono, ninte2, &d_tw));
+    const size_t na2 = (size_t)nA * nA;
+    uint64_t h = 1469598103934665603ull;
+    const int dims[4] = {N, nI, nA, ndimx};
+    h = fnv(h, dims, sizeof(dims));
+    h = fnv(h, &twono, sizeof(twono));
+    h = fnv(h, occ, (size_t)N * 8);
+    h = fnv(h, igem, (size_t)N * 4);
+    h = fnv(h, xone, (size_t)N * (N + 1) / 2 * 8);
+    h = ure ? fnv(h, ure, (size_t)N * N * 8) : fnv(h, "I", 1);
+    if (nA > 0) h = fnv(h, rdm2act, na2 * (na2 + 1) / 2 * 8);
+    h = fnv(h, indn, (size_t)2 * ndimx * 4);
+    if (st.sys_valid && st.sys_key == h && st.resident_key == twono) {
+        *out = &g_run;
+        return 0;
+    }
+    st.sys_valid = false;
+    GMC_TRY(stage_small(s, st.sysio, N, nA, ndimx, occ, xone, ure, nA > 0 ? rdm2act : nullptr, igem, indn,
+                        g_staged));
+    AcRun& r = g_run;
+    GMC_TRY(ac_prepare(s, N, nI, nA, ndimx, d_tw, g_staged.occ, g_staged.igem, g_staged.xone,
+                       ure ? g_staged.ure : nullptr, g_staged.rdm2act, g_staged.indn, r));
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    GMC_HIP(hipStreamSynchronize(s));
+    st.sys_key = h;
+    st.sys_valid = true;
+    *out = &r;
+    return 0;
+}
+
+// out = (1 - alpha) x0 + alpha x1, elementwise (A+-B of AB_CAS at alpha).
+__global__ void affine_kernel(const double* x0, const double* x1, double alpha, double* out, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (1.0 - alpha) * x0[i] + alpha * x1[i];
+}
+
+// FP64 ERPA of one alpha point with the reference's side outputs.
+static int erpa_one_fp64(hipStream_t s, int n, const AbPair& ab, double alpha, const double* Gw,
+                         const double* wdiag, double* d_Y, double* d_S, double* d_Am, double* eig_host, double* W_host,
+                         int* nneg_host)
+{
+    State& st = state();
+    EigPlan<double>* ep = nullptr;
+    GMC_TRY(st.eig_plan_for<double>(n, 1, &ep));
+    GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+    GMC_TRY(eig_bind<double>(ep, st.scratch.ptr));
+    ErpaPlan<double> pl;
+    GMC_TRY(erpa_plan<double>(n, 1, &pl));
+    pl.eig = ep;
+    const size_t outb = carve_bytes({8, (size_t)n * 8, 4});
+    GMC_TRY(st.erpaws.reserve(align_up(pl.ws_bytes) + outb));
+    char* base = static_cast<char*>(st.erpaws.ptr) + align_up(pl.ws_bytes);
+    ErpaOut out;
+    out.W = reinterpret_cast<double*>(base);
+    out.eig = reinterpret_cast<double*>(base + align_up(8));
+    out.nneg = reinterpret_cast<int*>(base + align_up(8) + align_up((size_t)n * 8));
+    GMC_TRY(erpa_batched<double>(s, pl, ab, &alpha, 1, Gw, wdiag, st.erpaws.ptr, out, d_Y, d_S, d_Am));
+    GMC_TRY(eig_check_error<double>(s, *ep));
+    if (eig_host) GMC_HIP(hipMemcpyAsync(eig_host, out.eig, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    if (W_host && Gw) GMC_HIP(hipMemcpyAsync(W_host, out.W, 8, hipMemcpyDeviceToHost, s));
+    if (nneg_host) GMC_HIP(hipMemcpyAsync(nneg_host, out.nneg, 4, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace gmc
+
+using namespace gmc;
+
+extern "C" {
+
+int gmc_twono1(double* tno, const double* ure, int nbasis, int64_t ninte2)
+{
+    GMC_TRY(ensure_init());
+    State& st = state();
+    hipStream_t s = st.stream;
+    st.resident_key = nullptr;
+    st.sys_valid = false;
+    const size_t bytes = (size_t)ninte2 * sizeof(double), ub = (size_t)nbasis * nbasis * sizeof(double);
+    GMC_TRY(st.resident.reserve(bytes));
+    GMC_TRY(st.io.reserve(ub));
+    GMC_HIP(hipMemcpyAsync(st.resident.ptr, tno, bytes, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(st.io.ptr, ure, ub, hipMemcpyHostToDevice, s));
+    GMC_TRY(gmc_twono1_dev(st.resident.as<double>(), st.io.as<double>(), nbasis, ninte2, s));
+    GMC_HIP(hipMemcpyAsync(tno, st.resident.ptr, bytes, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    // the transformed integrals stay resident for the AC-CAS calls that follow
+    st.resident_key = tno;
+    st.resident_n = ninte2;
+    probe_values(tno, ninte2, st.resident_probe);
+    return 0;
+}
+
+int gmc_forget_resident(void)
+{
+    State& st = state();
+    st.resident_key = nullptr;
+    st.resident_n = 0;
+    st.sys_valid = false;
+    return 0;
+}
+
+int gmc_ab_cas(double* abplus, double* abmin, double* etot, const double* ure, const double* occ, const double* xone,
+               const double* twono, int64_t ninte2, const int* indn, int ndimx, int nbasis, int ninact, int nact,
+               const int* igem, const double* rdm2act, double acalpha, double* cicoef)
+{
+    GMC_TRY(ensure_init());
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
+    State& st = state();
+    const size_t nn = (size_t)ndimx * ndimx;
+    GMC_TRY(st.io.reserve(carve_bytes({nn * 8, nn * 8})));
+    Arena ar{static_cast<char*>(st.io.ptr), carve_bytes({nn * 8, nn * 8})};
+    double* P = ar.take<double>(nn);
+    double* M = ar.take<double>(nn);
+    const unsigned g = (unsigned)std::min<size_t>(65536, (nn + 255) / 256);
+    hipLaunchKernelGGL(affine_kernel, dim3(g), dim3(256), 0, s, r->ab.p0, r->ab.p1, acalpha, P, (int64_t)nn);
+    hipLaunchKernelGGL(affine_kernel, dim3(g), dim3(256), 0, s, r->ab.m0, r->ab.m1, acalpha, M, (int64_t)nn);
+    GMC_HIP(hipGetLastError());
+    GMC_HIP(hipMemcpyAsync(abplus, P, nn * 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipMemcpyAsync(abmin, M, nn * 8, hipMemcpyDeviceToHost, s));
+    if (cicoef) GMC_HIP(hipMemcpyAsync(cicoef, r->ab.cicoef, (size_t)nbasis * 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    if (etot) *etot = r->ab.ecasscf;
+    return 0;
+}
+
+int gmc_erpasymm1(double* eigvecr, double* eig, double* aplsqrt, double* abmin, int ndimx, int* nneg)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(ndimx >= 1, "ERPASYMM1: NDimX=%d", ndimx);
+    State& st = state();
+    hipStream_t s = st.stream;
+    const size_t nn = (size_t)ndimx * ndimx;
+    const size_t bytes = carve_bytes({nn * 8, nn * 8, nn * 8, nn * 8, nn * 8});
+    GMC_TRY(st.io.reserve(bytes));
+    Arena ar{static_cast<char*>(st.io.ptr), bytes};
+    double* P = ar.take<double>(nn);
+    double* M = ar.take<double>(nn);
+    double* Y = ar.take<double>(nn);
+    double* S = ar.take<double>(nn);
+    double* Ms = ar.take<double>(nn);
+    GMC_HIP(hipMemcpyAsync(P, aplsqrt, nn * 8, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(M, abmin, nn * 8, hipMemcpyHostToDevice, s));
+    // alpha = 0 with A(0) = A(1): form reproduces the inputs exactly
+    AbPair ab{P, M, P, M, nullptr, 0.0};
+    GMC_TRY(erpa_one_fp64(s, ndimx, ab, 0.0, nullptr, nullptr, Y, S, Ms, eig, nullptr, nneg));
+    GMC_HIP(hipMemcpyAsync(eigvecr, Y, nn * 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipMemcpyAsync(aplsqrt, S, nn * 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipMemcpyAsync(abmin, Ms, nn * 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gmc_aceneerpa(double* ecorr, const double* eigvecr, const double* eig, const double* twono, int64_t ninte2,
+                  const double* occ, const int* igem, const double* cicoef, const int* indn, int nbasis, int ndimx)
+{
+    GMC_TRY(ensure_init());
+    State& st = state();
+    hipStream_t s = st.stream;
+    const int N = nbasis;
+    GMC_CHECK(ninte2 == (int64_t)N * (N + 1) / 2 * ((int64_t)N * (N + 1) / 2 + 1) / 2,
+              "ACEneERPA: NInte2=%lld does not match NBasis=%d", (long long)ninte2, N);
+    const double* d_tw = nullptr;
+    GMC_TRY(resident_twono(s, twono, ninte2, &d_tw));
+    const size_t nn = (size_t)ndimx * ndimx;
+    const size_t bytes = carve_bytes({nn * 8, (size_t)ndimx * 8, (size_t)N * 8, (size_t)N * 8, (size_t)N * 4,
+                                      (size_t)2 * ndimx * 4, 8, 8, nn * 8, integrand_from_y_bytes(ndimx)});
+    GMC_TRY(st.io.reserve(bytes));
+    Arena ar{static_cast<char*>(st.io.ptr), bytes};
+    double* Y = ar.take<double>(nn);
+    double* e = ar.take<double>(ndimx);
+    double* d_occ = ar.take<double>(N);
+    double* d_c = ar.take<double>(N);
+    int* d_g = ar.take<int>(N);
+    int* d_ind = ar.take<int>((size_t)2 * ndimx);
+    double* wdiag = ar.take<double>(1);
+    double* W = ar.take<double>(1);
+    double* Gw = ar.take<double>(nn);
+    char* scr = ar.take<char>(integrand_from_y_bytes(ndimx));
+    GMC_CHECK(ar.fits(), "ACEneERPA: staging layout overflow");
+    GMC_HIP(hipMemcpyAsync(Y, eigvecr, nn * 8, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(e, eig, (size_t)ndimx * 8, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(d_occ, occ, (size_t)N * 8, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(d_c, cicoef, (size_t)N * 8, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(d_g, igem, (size_t)N * 4, hipMemcpyHostToDevice, s));
+    GMC_HIP(hipMemcpyAsync(d_ind, indn, (size_t)2 * ndimx * 4, hipMemcpyHostToDevice, s));
+    AcSystem sys;
+    sys.N = N;
+    sys.ndimx = ndimx;
+    sys.twono = d_tw;
+    sys.occ = d_occ;
+    sys.igem = d_g;
+    sys.indn = d_ind;
+    GMC_TRY(integrand_weights(s, sys, d_c, Gw, wdiag));
+    GMC_TRY(integrand_from_y(s, ndimx, Y, e, Gw, wdiag, scr, W));
+    GMC_HIP(hipMemcpyAsync(ecorr, W, 8, hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, int64_t ninte2, const double* ure,
+                 const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
+                 int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
+                 double* eigvecr, double* eig)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACEInteg: unknown precision mode %d", prec);
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
+    double W = 0.0;
+    const bool side = abplus || abmin || eigvecr || eig;
+    if (prec == GMC_FP64 && side) {
+        State& st = state();
+        const size_t nn = (size_t)ndimx * ndimx;
+        const size_t bytes = carve_bytes({nn * 8, nn * 8, nn * 8});
+        GMC_TRY(st.io.reserve(bytes));
+        Arena ar{static_cast<char*>(st.io.ptr), bytes};
+        double* Y = ar.take<double>(nn);
+        double* S = ar.take<double>(nn);
+        double* Ms = ar.take<double>(nn);
+        std::vector<double> e(ndimx);
+        GMC_TRY(erpa_one_fp64(s, ndimx, r->ab, acalpha, r->Gw, r->wdiag, Y, S, Ms, e.data(), &W, nullptr));
+        if (eigvecr) GMC_HIP(hipMemcpyAsync(eigvecr, Y, nn * 8, hipMemcpyDeviceToHost, s));
+        if (abplus) GMC_HIP(hipMemcpyAsync(abplus, S, nn * 8, hipMemcpyDeviceToHost, s));
+        if (abmin) GMC_HIP(hipMemcpyAsync(abmin, Ms, nn * 8, hipMemcpyDeviceToHost, s));
+        GMC_HIP(hipStreamSynchronize(s));
+        if (eig) std::memcpy(eig, e.data(), (size_t)ndimx * 8);
+    } else {
+        GMC_TRY(ac_alpha_batches(s, *r, prec, 1, 1, &acalpha, &W, nullptr, nullptr));
+    }
+    *ecorr = W;
+    if (egone) *egone = r->ab.ecasscf;
+    return 0;
+}
+
+int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono, int64_t ninte2,
+                const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
+                int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
+                double* walpha, double* ecasscf, int* nneg)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(nalpha >= 1, "ACECORR: no alpha points");
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR: unknown precision mode %d", prec);
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
+    std::vector<double> w(nalpha);
+    GMC_TRY(ac_alpha_batches(s, *r, prec, alpha_batch > 0 ? alpha_batch : 16, nalpha, alphas, w.data(), nullptr,
+                             nneg));
+    double e = 0.0;
+    for (int i = 0; i < nalpha; i++) e += weights[i] * w[i];   // acfd.f:199
+    if (walpha)
+        for (int i = 0; i < nalpha; i++) walpha[i] = w[i];
+    if (ecorr) *ecorr = e;
+    if (ecasscf) *ecasscf = r->ab.ecasscf;
+    return 0;
+}
+
+}  // extern "C"

@@ -18,9 +18,13 @@ struct State {
     Workspace resident;    // integrals kept on the device across alpha points
     Workspace acws;        // A(0), A(1), integrand weights, assembly scratch
     Workspace erpaws;      // per-alpha ERPA matrices
+    Workspace sysio;       // small arrays of the system cached for host-pointer calls
+    Workspace io;          // host <-> device staging of host-pointer calls
     const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
     int64_t resident_n = 0;
-    double resident_probe[8] = {0};
+    double resident_probe[64] = {0};
+    bool sys_valid = false;               // acws/sysio hold the system hashed as sys_key
+    uint64_t sys_key = 0;
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<double>>> eig64;
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<float>>> eig32;
     template <class T>
@@ -32,12 +36,17 @@ struct State {
         if (resident.ptr) hipFree(resident.ptr);
         if (acws.ptr) hipFree(acws.ptr);
         if (erpaws.ptr) hipFree(erpaws.ptr);
+        if (sysio.ptr) hipFree(sysio.ptr);
+        if (io.ptr) hipFree(io.ptr);
         scratch = Workspace{};
         resident = Workspace{};
         acws = Workspace{};
         erpaws = Workspace{};
+        sysio = Workspace{};
+        io = Workspace{};
         resident_key = nullptr;
         resident_n = 0;
+        sys_valid = false;
     }
 };
 

@@ -139,3 +139,99 @@ def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False, stream=None,
     if return_eig:
         out["eig"] = eig
     return out
+
+
+# ---------------------------------------------------------------- host drop-ins
+# numpy (host) arrays in and out, the reference's argument meaning; COMMON
+# inputs (NInAcCAS, NAcCAS, IGem) and the rdm2.dat contents are explicit.
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def AB_CAS(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, acalpha):
+    """A+B, A-B at ACAlpha (interpa.f:3659): (ABPLUS, ABMIN, ETot, CICoef),
+    NDimX x NDimX Fortran-ordered, not symmetrised."""
+    # every converted array is bound to a name: ctypes keeps only the address
+    indn, ig = _i(indn), _i(igem)
+    oc, xo, rd = _f(occ), _f(xone), _f(rdm2act)
+    u = None if ure is None else _f(ure)
+    nd = len(indn)
+    ap = np.zeros((nd, nd), order="F")
+    am = np.zeros((nd, nd), order="F")
+    et = np.zeros(1)
+    ci = np.zeros(nbasis)
+    _lib.call("gmc_ab_cas", ap.ctypes.data, am.ctypes.data, et.ctypes.data, _ptr(u), oc.ctypes.data, xo.ctypes.data,
+              twono.ctypes.data, len(twono), indn.ctypes.data, nd, nbasis, ninact, nact, ig.ctypes.data,
+              rd.ctypes.data, float(acalpha), ci.ctypes.data)
+    return ap, am, float(et[0]), ci
+
+
+def ERPASYMM1(abplus, abmin):
+    """ERPASYMM1 (interpa.f:1652): returns (EigVecR, Eig, APLSQRT, ABMIN
+    symmetrised, number of negative eigenvalues of A+B)."""
+    ap = _f(abplus).copy(order="F")
+    am = _f(abmin).copy(order="F")
+    nd = ap.shape[0]
+    y = np.zeros((nd, nd), order="F")
+    e = np.zeros(nd)
+    nneg = np.zeros(1, dtype=np.int32)
+    _lib.call("gmc_erpasymm1", y.ctypes.data, e.ctypes.data, ap.ctypes.data, am.ctypes.data, nd, nneg.ctypes.data)
+    return y, e, ap, am, int(nneg[0])
+
+
+def ACEneERPA(eigvecr, eig, twono, occ, igem, cicoef, indn, nbasis):
+    """W(alpha) from ERPASYMM1's Y and Eig (acfd.f:4968)."""
+    indn, ig = _i(indn), _i(igem)
+    y, e, oc, ci = _f(eigvecr), _f(eig), _f(occ), _f(cicoef)
+    nd = len(indn)
+    ec = np.zeros(1)
+    _lib.call("gmc_aceneerpa", ec.ctypes.data, y.ctypes.data, e.ctypes.data, twono.ctypes.data, len(twono),
+              oc.ctypes.data, ig.ctypes.data, ci.ctypes.data, indn.ctypes.data, nbasis, nd)
+    return float(ec[0])
+
+
+def ACEInteg(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, acalpha, prec=GMC_FP64,
+             side_outputs=False):
+    """One alpha point (acfd.f:4832): returns dict(ecorr=W(alpha), egone=ECASSCF
+    and, with side_outputs, ABPLUS=(A+B)^(1/2), ABMIN, EigVecR, Eig)."""
+    indn, ig = _i(indn), _i(igem)
+    oc, xo, rd = _f(occ), _f(xone), _f(rdm2act)
+    u = None if ure is None else _f(ure)
+    nd = len(indn)
+    ec = np.zeros(1)
+    eg = np.zeros(1)
+    outs = [np.zeros((nd, nd), order="F") for _ in range(3)] + [np.zeros(nd)] if side_outputs else [None] * 4
+    _lib.call("gmc_aceinteg", prec, ec.ctypes.data, eg.ctypes.data, twono.ctypes.data, len(twono), _ptr(u),
+              oc.ctypes.data, xo.ctypes.data, ig.ctypes.data, rd.ctypes.data, indn.ctypes.data, nd, nbasis, ninact,
+              nact, float(acalpha), *[_ptr(o) for o in outs])
+    res = dict(ecorr=float(ec[0]), egone=float(eg[0]))
+    if side_outputs:
+        res.update(abplus=outs[0], abmin=outs[1], eigvecr=outs[2], eig=outs[3])
+    return res
+
+
+def ACECORR_host(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, alphas, weights,
+                 prec=GMC_FP64, alpha_batch=0):
+    """ACECORR's alpha loop with host inputs (acfd.f:161-201)."""
+    indn, ig = _i(indn), _i(igem)
+    oc, xo, rd = _f(occ), _f(xone), _f(rdm2act)
+    u = None if ure is None else _f(ure)
+    al = np.ascontiguousarray(alphas, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    na = len(al)
+    ec, es = np.zeros(1), np.zeros(1)
+    wa = np.zeros(na)
+    nneg = np.zeros(na, dtype=np.int32)
+    _lib.call("gmc_acecorr", prec, alpha_batch, nbasis, ninact, nact, twono.ctypes.data, len(twono), oc.ctypes.data,
+              ig.ctypes.data, xo.ctypes.data, _ptr(u), rd.ctypes.data, len(indn), indn.ctypes.data, na,
+              al.ctypes.data, wt.ctypes.data, ec.ctypes.data, wa.ctypes.data, es.ctypes.data, nneg.ctypes.data)
+    return dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define GMC_ABI_VERSION 1
+#define GMC_ABI_VERSION 2
 /* arithmetic modes of the AC-CAS solve */
 #define GMC_FP64 0    /* plain double precision throughout                     */
 #define GMC_FP32 1    /* eigen solves in float, no refinement (diagnostic)     */
@@ -35,9 +35,66 @@ int gmc_finalize(void);
 /* TwoNO1(TNO, URe, NBasis, NInte2) - misc.f:166.  In-place 4-index
  * transformation of the 8-fold packed two-electron integrals (NAddr3 order,
  * misc.f:505) with URe(NBasis,NBasis) = UAOMO(NO,AO).  int64 NInte2 lifts the
- * reference's 32-bit limit. */
+ * reference's 32-bit limit.  The host form leaves the transformed integrals
+ * resident on the device for the AC-CAS calls below. */
 int gmc_twono1(double* tno, const double* ure, int nbasis, int64_t ninte2);
 int gmc_twono1_dev(double* tno, const double* ure, int nbasis, int64_t ninte2, void* stream);
+
+/* ---- host-pointer drop-ins of the reference subroutines ------------------
+ * Column-major Fortran arrays, IndN 1-based.  The COMMON inputs of the
+ * reference routines are explicit arguments: ninact = NInAcCAS, nact = NAcCAS
+ * (/CAS/), igem = IGem(1:NBasis) (/APSG/), and rdm2act is the packed active
+ * 2-RDM that AB_CAS reads from rdm2.dat (interpa.f:3734-3741), nA^2(nA^2+1)/2
+ * values in NAddrRDM order.  The packed integrals twono (NInte2) are kept on
+ * the device across calls, keyed by their address, size and a sample of
+ * values; A(0), A(1) and the integrand weights of the last system are kept
+ * too (keyed by a hash of its small arrays).  A caller that rewrites twono in
+ * place between calls must call gmc_forget_resident() first. */
+
+/* AB_CAS(ABPLUS,ABMIN,ETot,URe,Occ,XOne,TwoNO,IPair,IndN,IndX,NDimX,NBasis,
+ * NDim,NInte1,NInte2,ACAlpha) - interpa.f:3659-3660.  abplus/abmin: NDimX^2
+ * (not symmetrised, as the reference leaves them); etot = ECASSCF without
+ * ENuc; cicoef (NBasis, may be NULL) = the CICoef the reference stores in
+ * /APSG/ (interpa.f:3777-3781).  IndX must be the identity (ACCAS). */
+int gmc_ab_cas(double* abplus, double* abmin, double* etot, const double* ure, const double* occ, const double* xone,
+               const double* twono, int64_t ninte2, const int* indn, int ndimx, int nbasis, int ninact, int nact,
+               const int* igem, const double* rdm2act, double acalpha, double* cicoef);
+
+/* ERPASYMM1(EigVecR,Eig,APLSQRT,ABMIN,NBasis,NDimX) - interpa.f:1652.  FP64.
+ * In: aplsqrt = A+B, abmin = A-B (NDimX^2).  Out: eigvecr = normalised Y
+ * columns, eig = omega ascending (omega^2 where omega^2 <= 1e-6), aplsqrt =
+ * (A+B)^(1/2) with |lambda| (interpa.f:1695-1716), abmin symmetrised; nneg
+ * (may be NULL) = negative eigenvalues of A+B. */
+int gmc_erpasymm1(double* eigvecr, double* eig, double* aplsqrt, double* abmin, int ndimx, int* nneg);
+
+/* ACEneERPA(ECorr,EigVecR,Eig,TwoNO,URe,Occ,XOne,IndN,NBasis,NInte1,NInte2,
+ * NDimX,NGem) - acfd.f:4968-4970.  ecorr = W(alpha) from the Y columns of
+ * ERPASYMM1 with 1e-3 < Eig < 1e8; cicoef is /APSG/ CICoef as AB_CAS set it. */
+int gmc_aceneerpa(double* ecorr, const double* eigvecr, const double* eig, const double* twono, int64_t ninte2,
+                  const double* occ, const int* igem, const double* cicoef, const int* indn, int nbasis, int ndimx);
+
+/* ACEInteg(ECorr,TwoNO,URe,Occ,XOne,UNOAO,ABPLUS,ABMIN,EigVecR,Eig,EGOne,
+ * NGOcc,Title,NBasis,NInte1,NInte2,NDim,NGem,IndAux,ACAlpha,IndN,IndX,NDimX)
+ * - acfd.f:4832-4836, its CASSCF branch with NoSt = 1: AB_CAS -> ERPASYMM1 ->
+ * ACEneERPA at one alpha.  ecorr = W(alpha), egone = EGOne(1) = ECASSCF.
+ * abplus, abmin, eigvecr (NDimX^2) and eig (NDimX) receive ERPASYMM1's
+ * outputs when non-NULL and prec = GMC_FP64 (left untouched otherwise). */
+int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, int64_t ninte2, const double* ure,
+                 const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
+                 int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
+                 double* eigvecr, double* eig);
+
+/* ACECORR's alpha loop (acfd.f:161-201) with host inputs: all alpha points
+ * batched (alpha_batch points per solve, 0 = 16), ecorr = sum w_i W(alpha_i),
+ * walpha (nalpha, may be NULL), ecasscf, nneg (nalpha, may be NULL). */
+int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono, int64_t ninte2,
+                const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
+                int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,
+                double* walpha, double* ecasscf, int* nneg);
+
+/* Drop the resident integrals and the cached system (no reference
+ * counterpart; for callers that modify TwoNO in place between calls). */
+int gmc_forget_resident(void);
 
 /* Building block exported for tests: batched column-major GEMM on MFMA,
  * C_b = alpha op(A_b) op(B_b) + beta C_b.  prec = GMC_FP64 or GMC_FP32 (then

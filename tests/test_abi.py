@@ -15,7 +15,8 @@ def declared():
 
 def test_header_declares_the_drop_in_entry_points():
     names = declared()
-    for n in ("gmc_twono1", "gmc_acecorr_dev", "gmc_last_error"):
+    for n in ("gmc_twono1", "gmc_ab_cas", "gmc_erpasymm1", "gmc_aceneerpa", "gmc_aceinteg", "gmc_acecorr",
+              "gmc_acecorr_dev", "gmc_last_error"):
         assert n in names
 
 
@@ -30,4 +31,4 @@ def test_library_exports_every_declared_symbol():
 def test_version_without_gpu():
     from gammcor_amd import _lib
     lib = ctypes.CDLL(_lib.LIB_PATH)
-    assert lib.gmc_version() == 1
+    assert lib.gmc_version() == 2
