@@ -348,7 +348,18 @@ int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, in
         GMC_HIP(hipStreamSynchronize(s));
         if (eig) std::memcpy(eig, e.data(), (size_t)ndimx * 8);
     } else {
-        GMC_TRY(ac_alpha_batches(s, *r, prec, 1, 1, &acalpha, &W, nullptr, nullptr));
+        // other modes return W and Eig; the matrices stay untouched
+        State& st = state();
+        double* d_eig = nullptr;
+        if (eig) {
+            GMC_TRY(st.io.reserve((size_t)ndimx * 8));
+            d_eig = st.io.as<double>();
+        }
+        GMC_TRY(ac_alpha_batches(s, *r, prec, 1, 1, &acalpha, &W, d_eig, nullptr));
+        if (eig) {
+            GMC_HIP(hipMemcpyAsync(eig, d_eig, (size_t)ndimx * 8, hipMemcpyDeviceToHost, s));
+            GMC_HIP(hipStreamSynchronize(s));
+        }
     }
     *ecorr = W;
     if (egone) *egone = r->ab.ecasscf;

@@ -78,7 +78,8 @@ int gmc_aceneerpa(double* ecorr, const double* eigvecr, const double* eig, const
  * - acfd.f:4832-4836, its CASSCF branch with NoSt = 1: AB_CAS -> ERPASYMM1 ->
  * ACEneERPA at one alpha.  ecorr = W(alpha), egone = EGOne(1) = ECASSCF.
  * abplus, abmin, eigvecr (NDimX^2) and eig (NDimX) receive ERPASYMM1's
- * outputs when non-NULL and prec = GMC_FP64 (left untouched otherwise). */
+ * outputs when non-NULL; in the modes other than GMC_FP64 only eig is
+ * written and the three matrices are left untouched. */
 int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, int64_t ninte2, const double* ure,
                  const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
                  int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
