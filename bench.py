@@ -134,6 +134,7 @@ def main():
     PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12}
     KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
                "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
+               "blas_gemm_f64": ("mfma", PEAK["mfma_f64"]), "blas_gemm_f32": ("mfma", PEAK["mfma_f32"]),
                "twono1_sandwich": ("mfma", PEAK["mfma_f64"]), "ab_cas_main": ("hbm", PEAK["hbm"])}
 
     def timed(prec, ab, steps, warmup, profile=False):

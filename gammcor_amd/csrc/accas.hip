@@ -565,26 +565,26 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     GMC_HIP(hipGetLastError());
     // S = V|L|^(1/2)V^T
     GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
-    GMC_TRY(gemm<T>(st, gS, T(1), E, E, T(0), Sb));
+    GMC_TRY(gemm_big<T>(st, gS, T(1), E, E, T(0), Sb));
     if (S_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: A+^(1/2) output is available from the FP64 solve");
         GMC_HIP(hipMemcpyAsync(S_out, Sb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
     // M = S A- S
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
-    GMC_TRY(gemm<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
-    GMC_TRY(gemm<T>(st, gN, T(1), Sb, Mb, T(0), Tb));
+    GMC_TRY(gemm_big<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
+    GMC_TRY(gemm_big<T>(st, gN, T(1), Sb, Mb, T(0), Tb));
     hipLaunchKernelGGL(symmetrize_kernel<T>, gsq, blk, 0, st, Tb, n);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<T>(st, ep, Tb, om2));
     // Y = S Z
-    GMC_TRY(gemm<T>(st, gN, T(1), Sb, Tb, T(0), Mb));
+    GMC_TRY(gemm_big<T>(st, gN, T(1), Sb, Tb, T(0), Mb));
     hipLaunchKernelGGL(ycols_kernel<T>, gsq, blk, 0, st, Mb, E, om2, out.eig, n);
     GMC_HIP(hipGetLastError());
     // P = Yf Yf^T (lower) and W (skipped without integrand weights)
     if (Gw) {
         GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-        GMC_TRY(gemm<T>(st, gP, T(1), E, E, T(0), Sb));
+        GMC_TRY(gemm_big<T>(st, gP, T(1), E, E, T(0), Sb));
         hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
         hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
         GMC_HIP(hipGetLastError());
@@ -768,9 +768,9 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GemmShape gtl{n, n, n, true, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-    GMC_TRY(gemm<double>(st, gnn, 1.0, A, X, 0.0, AX));
-    GMC_TRY(gemm<double>(st, gtl, 1.0, X, AX, 0.0, S));
-    GMC_TRY(gemm<double>(st, gtl, 1.0, X, X, 0.0, G));
+    GMC_TRY(gemm_big<double>(st, gnn, 1.0, A, X, 0.0, AX));
+    GMC_TRY(gemm_big<double>(st, gtl, 1.0, X, AX, 0.0, S));
+    GMC_TRY(gemm_big<double>(st, gtl, 1.0, X, X, 0.0, G));
     hipLaunchKernelGGL(oa_stats_kernel, dim3(npart, B), dim3(256), 0, st, S, G, n, lam, partial);
     hipLaunchKernelGGL(oa_delta_kernel, dim3(1), dim3(64), 0, st, partial, npart, delta, B);
     GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
@@ -778,7 +778,7 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
                        op.pairs, op.rot);
     GMC_HIP(hipGetLastError());
     GMC_HIP(hipMemcpyAsync(Xn, X, B * nn * sizeof(double), hipMemcpyDeviceToDevice, st));
-    GMC_TRY(gemm<double>(st, gnn, 1.0, X, AX, 1.0, Xn));
+    GMC_TRY(gemm_big<double>(st, gnn, 1.0, X, AX, 1.0, Xn));
     std::swap(X, Xn);
     hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot);
     hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot);
@@ -864,10 +864,10 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
     GMC_HIP(hipGetLastError());
     GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
-    GMC_TRY(gemm<double>(st, gS, 1.0, X, X, 0.0, AX));
+    GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
-    GMC_TRY(gemm<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
-    GMC_TRY(gemm<double>(st, gN, 1.0, AX, Sb, 0.0, Ap));
+    GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
+    GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, Sb, 0.0, Ap));
     hipLaunchKernelGGL(symmetrize_kernel<double>, gsq, blk, 0, st, Ap, n);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
@@ -880,10 +880,10 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         GMC_TRY(report("M", it));
     }
     // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
-    GMC_TRY(gemm<double>(st, gN, 1.0, AX, X, 0.0, Sb));
+    GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
     hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);
     GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-    GMC_TRY(gemm<double>(st, gP, 1.0, Gb, Gb, 0.0, Am));
+    GMC_TRY(gemm_big<double>(st, gP, 1.0, Gb, Gb, 0.0, Am));
     hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, B), blk, 0, st, Gw, Am, n, partial);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
     GMC_HIP(hipGetLastError());
@@ -923,7 +923,7 @@ int integrand_from_y(hipStream_t st, int n, const double* Y, const double* eig, 
     GMC_CHECK(ar.fits(), "ACEneERPA: scratch layout overflow");
     hipLaunchKernelGGL(filter_cols_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, Y, eig, Yf, n);
     GemmShape gP{n, n, n, false, true, n, n, n, 0, 0, 0, 1, 1};
-    GMC_TRY(gemm<double>(st, gP, 1.0, Yf, Yf, 0.0, P));
+    GMC_TRY(gemm_big<double>(st, gP, 1.0, Yf, Yf, 0.0, P));
     hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, 1), dim3(256), 0, st, Gw, P, n, partial);
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, W, 1);
     GMC_HIP(hipGetLastError());

@@ -81,7 +81,7 @@ __device__ inline void group_barrier(unsigned* flags, int gi, int G, unsigned& e
         long spins = 0;
         while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++spins > (1L << 26)) {
+            if (++spins > (1L << 22)) {   // seconds of polling: a lost workgroup
                 atomicExch(err, 1);
                 break;
             }
@@ -1156,6 +1156,22 @@ static inline unsigned grid1(int64_t work, int per = 256, int64_t cap = 1 << 20)
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
+// The panel kernel needs its B*G workgroups (at most one per CU) resident
+// together.  A cooperative launch guarantees it; when the runtime refuses one
+// (its global wave-sync resource is held by another process on the device),
+// a plain launch of the same grid is used: with the grid no larger than the
+// CU count every workgroup is placed at once on an otherwise idle device, and
+// the kernel's bounded barrier spin turns a lost co-residency into an error
+// (reported by eig_check_error) instead of a hang.
+static int launch_persistent(const void* kfn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s)
+{
+    hipError_t e = hipLaunchCooperativeKernel(kfn, grid, block, args, (unsigned)lds, s);
+    if (e == hipSuccess) return 0;
+    (void)hipGetLastError();
+    GMC_HIP(hipLaunchKernel(kfn, grid, block, args, lds, s));
+    return 0;
+}
+
 template <class T>
 int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
 {
@@ -1193,7 +1209,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             for (int jj = k; jj < k + kb; jj++) bytes += 0.5 * (double)(n - jj - 1) * (double)(n - jj) * sizeof(T);
             {
                 ProfScope ps(s, sizeof(T) == 8 ? "sytrd_panel_f64" : "sytrd_panel_f32", bytes * B);
-                GMC_HIP(hipLaunchCooperativeKernel(kfn, dim3(B * G), dim3(SY_THREADS), args, (unsigned)lds, s));
+                GMC_TRY(launch_persistent(kfn, dim3(B * G), dim3(SY_THREADS), args, lds, s));
             }
             epoch += 3 * kb;
             const int k1 = k + kb;

@@ -20,6 +20,12 @@ struct GemmShape {
 template <class T>
 int gemm(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
 
+// Large plain products (rocBLAS, blas.hip).  The whole of C is written even
+// when g.uplo asks for the lower triangle only.
+template <class T>
+int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
+void blas_release(void* handle);
+
 // Same, with one pointer per batch entry (device arrays of pointers).
 template <class T>
 int gemm_ptr(hipStream_t s, const GemmShape& g, T alpha, const T* const* A, const T* const* B, T beta,

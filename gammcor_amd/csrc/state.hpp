@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 #include "eigen.hpp"
+#include "gemm.hpp"
 
 namespace gmc {
 
@@ -25,6 +26,7 @@ struct State {
     double resident_probe[64] = {0};
     bool sys_valid = false;               // acws/sysio hold the system hashed as sys_key
     uint64_t sys_key = 0;
+    void* blas = nullptr;                 // rocBLAS handle (blas.hip)
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<double>>> eig64;
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<float>>> eig32;
     template <class T>
@@ -32,6 +34,8 @@ struct State {
     void release() {
         eig64.clear();
         eig32.clear();
+        blas_release(blas);
+        blas = nullptr;
         if (scratch.ptr) hipFree(scratch.ptr);
         if (resident.ptr) hipFree(resident.ptr);
         if (acws.ptr) hipFree(acws.ptr);
