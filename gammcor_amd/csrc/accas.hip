@@ -762,8 +762,20 @@ struct OaPairs {
     double2* rot;       // B x OA_MAXPAIRS
 };
 
+// Xn = X + P, P the FP32 product X E of the update.
+__global__ void oa_add_kernel(const double* X, const float* P, double* Xn, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        Xn[i] = X[i] + (double)P[i];
+}
+
+// The update X E is formed in FP32: every |E_ij| <= 1/8 and the product is a
+// correction of X, so its FP32 rounding (~1e-7 |X||E|) stays far below what
+// the step removes and the next step corrects it; the residual terms
+// (A X, X^T A X, X^T X) are FP64.
 static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
-                     double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op)
+                     double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
+                     float* E32, float* P32)
 {
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
@@ -777,8 +789,13 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
                        op.pairs, op.rot);
     GMC_HIP(hipGetLastError());
-    GMC_HIP(hipMemcpyAsync(Xn, X, B * nn * sizeof(double), hipMemcpyDeviceToDevice, st));
-    GMC_TRY(gemm_big<double>(st, gnn, 1.0, X, AX, 1.0, Xn));
+    const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, X, X32, (int64_t)B * nn);
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, AX, E32, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
+    GMC_TRY(gemm_big<float>(st, gnn, 1.0f, X32, E32, 0.0f, P32));
+    hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, (int64_t)B * nn);
+    GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
     hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot);
     hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot);
@@ -800,7 +817,7 @@ size_t erpa_mixed_bytes(int n, int batch)
     const size_t nn = (size_t)n * n, B = batch;
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
-                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16});
+                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4});
 }
 
 // Mixed ERPA: eigen solves of A+ and of M = S A- S in FP32, each refined in
@@ -832,6 +849,8 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     op.npair = ar.take<unsigned>(B);
     op.pairs = ar.take<int4>((size_t)B * OA_MAXPAIRS);
     op.rot = ar.take<double2>((size_t)B * OA_MAXPAIRS);
+    float* X32 = ar.take<float>(B * nn);
+    float* P32 = ar.take<float>(B * nn);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
     auto report = [&](const char* what, int it) -> int {
@@ -857,7 +876,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     for (int it = 0; it < n_oa; it++) {
-        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, lam, AX, Sb, Gb, partial, npart, delta, op));
+        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, lam, AX, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
         GMC_TRY(report("A+", it));
     }
     // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
@@ -876,7 +895,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     for (int it = 0; it < n_oa; it++) {
-        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, om2, Am, Sb, Gb, partial, npart, delta, op));
+        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, om2, Am, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
         GMC_TRY(report("M", it));
     }
     // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand

@@ -41,10 +41,11 @@ int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B,
     GMC_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
     const rocblas_operation ta = g.ta ? rocblas_operation_transpose : rocblas_operation_none;
     const rocblas_operation tb = g.tb ? rocblas_operation_transpose : rocblas_operation_none;
-    // the whole product is formed even where only the lower triangle is asked for
+    rocblas_status rs;
+    // the whole product is formed even where only the lower triangle is asked
+    // for: rocBLAS's SYRK/GEMMT run at a fraction of its GEMM rate on gfx950
     const double flops = 2.0 * g.m * g.n * (double)g.k * g.batch;
     ProfScope ps(s, sizeof(T) == 8 ? "blas_gemm_f64" : "blas_gemm_f32", flops);
-    rocblas_status rs;
     if constexpr (sizeof(T) == 8)
         rs = rocblas_dgemm_strided_batched(h, ta, tb, g.m, g.n, g.k, &alpha, A, (rocblas_int)g.lda, g.sa, B,
                                            (rocblas_int)g.ldb, g.sb, &beta, C, (rocblas_int)g.ldc, g.sc, g.batch);

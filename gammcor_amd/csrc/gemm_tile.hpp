@@ -87,6 +87,22 @@ __device__ void gemm_tile(const GemmShape& g, int bi, int bj, T alpha, const T* 
         }
     };
 
+    // C tile read up front when it is accumulated into (beta != 0): its
+    // latency then overlaps the k loop instead of following it
+    T cpre[2][2][4];
+    if (beta != T(0)) {
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = bi + wi * 32 + x * 16 + Mfma<T>::row(lane, r);
+                    const int j = bj + wj * 32 + y * 16 + (lane & 15);
+                    cpre[x][y][r] = (i < g.m && j < g.n && (g.uplo == 0 || i >= j)) ? C[i + (int64_t)j * g.ldc]
+                                                                                       : T(0);
+                }
+    }
     load_tiles(0);
     for (int k0 = 0; k0 < g.k; k0 += BK) {
         __syncthreads();
@@ -116,10 +132,9 @@ __device__ void gemm_tile(const GemmShape& g, int bi, int bj, T alpha, const T* 
                 int i = bi + wi * 32 + x * 16 + Mfma<T>::row(lane, r);
                 int j = bj + wj * 32 + y * 16 + (lane & 15);
                 if (i < g.m && j < g.n && (g.uplo == 0 || i >= j)) {
-                    T* cp = C + i + (int64_t)j * g.ldc;
                     T v = alpha * acc[x][y][r];
-                    if (beta != T(0)) v += beta * *cp;
-                    *cp = v;
+                    if (beta != T(0)) v += beta * cpre[x][y][r];
+                    C[i + (int64_t)j * g.ldc] = v;
                 }
             }
 }
