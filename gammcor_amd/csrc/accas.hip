@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "accas.hpp"
@@ -677,13 +678,35 @@ __global__ void oa_delta_kernel(const double* partial, int np, double* delta, in
 // (lower index, upper index, Jacobi cosine and sine of its 2x2 block) so that
 // oa_pair_rot_kernel resolves it exactly after the update.
 constexpr int OA_MAXPAIRS = 4096;
+constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
+constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
 
-__global__ void oa_e_kernel(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                            unsigned* npair, int4* pairs, double2* rot)
+__device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+                          unsigned* npair, int4* pairs, double2* rot, int b, int i, int j);
+
+__global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
+                                                   const double* delta, double* E, int n, unsigned* npair, int4* pairs,
+                                                   double2* rot, unsigned* emax)
 {
     const int b = blockIdx.z;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
+    // largest off-diagonal correction of the block, one atomic per block
+    __shared__ float wmax[4];
+    float my = 0.f;
+    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, b, i, j);
+    for (int o = 32; o > 0; o >>= 1) my = fmaxf(my, __shfl_xor(my, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = my;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        atomicMax(&emax[(size_t)b * OA_EMAX_SLOTS + (blockIdx.y % OA_EMAX_SLOTS)], __float_as_uint(m));
+    }
+}
+
+// One element of E; returns |E_ij| for i != j (0 on the diagonal).
+__device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+                          unsigned* npair, int4* pairs, double2* rot, int b, int i, int j)
+{
     const size_t base = (size_t)b * n * n;
     const int hi = i > j ? i : j, lo = i > j ? j : i;
     const size_t at = base + hi + (size_t)lo * n;
@@ -713,6 +736,7 @@ __global__ void oa_e_kernel(const double* S, const double* G, const double* lam,
         }
     }
     E[base + i + (size_t)j * n] = e;
+    return i == j ? 0.f : (float)fabs(e);
 }
 
 // x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
@@ -760,6 +784,7 @@ struct OaPairs {
     unsigned* npair;    // B
     int4* pairs;        // B x OA_MAXPAIRS
     double2* rot;       // B x OA_MAXPAIRS
+    unsigned* emax;     // B x OA_EMAX_SLOTS: float bits of max |E_ij| (i != j) of the last step
 };
 
 // Xn = X + P, P the FP32 product X E of the update.
@@ -786,8 +811,9 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     hipLaunchKernelGGL(oa_stats_kernel, dim3(npart, B), dim3(256), 0, st, S, G, n, lam, partial);
     hipLaunchKernelGGL(oa_delta_kernel, dim3(1), dim3(64), 0, st, partial, npart, delta, B);
     GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
+    GMC_HIP(hipMemsetAsync(op.emax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
-                       op.pairs, op.rot);
+                       op.pairs, op.rot, op.emax);
     GMC_HIP(hipGetLastError());
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, X, X32, (int64_t)B * nn);
@@ -817,7 +843,8 @@ size_t erpa_mixed_bytes(int n, int batch)
     const size_t nn = (size_t)n * n, B = batch;
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
-                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4});
+                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4,
+                        B * (size_t)OA_EMAX_SLOTS * 4});
 }
 
 // Mixed ERPA: eigen solves of A+ and of M = S A- S in FP32, each refined in
@@ -851,6 +878,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     op.rot = ar.take<double2>((size_t)B * OA_MAXPAIRS);
     float* X32 = ar.take<float>(B * nn);
     float* P32 = ar.take<float>(B * nn);
+    op.emax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
     auto report = [&](const char* what, int it) -> int {
@@ -860,7 +888,31 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         GMC_HIP(hipStreamSynchronize(st));
         unsigned tot = 0;
         for (unsigned v : h) tot += v;
-        fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems\n", what, it, tot, B);
+        std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
+        GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        float mx = 0.f;
+        for (unsigned u : em) {
+            float f;
+            std::memcpy(&f, &u, 4);
+            mx = std::max(mx, f);
+        }
+        fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems, max |E| %.3e\n", what, it, tot, B, mx);
+        return 0;
+    };
+    // Stop once the last step's largest correction is below OA_TOL: the
+    // error left after it is of the order of its square.
+    auto converged = [&](int it, bool& done) -> int {
+        done = false;
+        if (it + 1 >= n_oa) return 0;
+        std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
+        GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        unsigned mx = 0;
+        for (unsigned u : em) mx = std::max(mx, u);   // non-negative floats order as their bits
+        float f;
+        std::memcpy(&f, &mx, 4);
+        done = f < OA_TOL;
         return 0;
     };
     GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
@@ -878,6 +930,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     for (int it = 0; it < n_oa; it++) {
         GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, lam, AX, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
         GMC_TRY(report("A+", it));
+        bool done = false;
+        GMC_TRY(converged(it, done));
+        if (done) break;
     }
     // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
     hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
@@ -897,6 +952,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     for (int it = 0; it < n_oa; it++) {
         GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, om2, Am, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
         GMC_TRY(report("M", it));
+        bool done = false;
+        GMC_TRY(converged(it, done));
+        if (done) break;
     }
     // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
     GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
