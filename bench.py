@@ -54,6 +54,22 @@ def run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, alpha_batch=0,
                         return_eig=return_eig)
 
 
+def load_traffic(config):
+    """HBM bytes per launch (FETCH_SIZE, WRITE_SIZE) of each kernel, from the
+    committed rocprofv3 --pmc summary of this workload, if there is one."""
+    name = f"accas_{config.split('_')[0]}_mixed_hbm_traffic.txt"
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
+    if os.path.exists(path):
+        out["_file"] = name
+        with open(path) as f:
+            for line in f:
+                t = line.split()
+                if t and t[0] == "TRAFFIC":
+                    out[t[1]] = (float(t[2]), float(t[3]))
+    return out
+
+
 def cpu_baseline(s, cfg, nalpha, ao_host):
     """The oracle (CPU restatement, numpy/LAPACK + C) timed on a bounded
     sample of the same workload: the transform's per-column cost measured on
@@ -134,8 +150,11 @@ def main():
     PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12}
     KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
                "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
-               "blas_gemm_f64": ("mfma", PEAK["mfma_f64"]), "blas_gemm_f32": ("mfma", PEAK["mfma_f32"]),
                "twono1_sandwich": ("mfma", PEAK["mfma_f64"]), "ab_cas_main": ("hbm", PEAK["hbm"])}
+    for ab in ("nn", "nt", "tn", "tt"):     # rocBLAS products, one kernel per transpose pair
+        KERNELS[f"blas_gemm_f64_{ab}"] = ("mfma", PEAK["mfma_f64"])
+        KERNELS[f"blas_gemm_f32_{ab}"] = ("mfma", PEAK["mfma_f32"])
+    traffic = load_traffic(args.config)
 
     def timed(prec, ab, steps, warmup, profile=False):
         for _ in range(warmup):
@@ -183,6 +202,13 @@ def main():
                 "work_definition": ("lower triangle of the trailing matrix read once per column, summed over the "
                                     "panel's columns and the alpha batch" if dom.startswith("sytrd") else
                                     "2mnk per product")}
+        if dom in traffic:
+            fe, wr = traffic[dom]
+            roof["traffic"] = fe + wr
+            roof["traffic_fetch"], roof["traffic_write"] = fe, wr
+            roof["traffic_over_algorithmic"] = (fe + wr) / per_launch if bound == "hbm" else None
+            roof["traffic_source"] = f"profiles/{traffic['_file']} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, " \
+                                     "calibrated by a 1 GiB copy)"
         roof["kernels"] = {k: dict(ms_per_step=v["ms"] / args.steps, launches=v["launches"],
                                    achieved=v["work"] / (v["ms"] * 1e-3) / (1e9 if KERNELS[k][0] == "hbm" else 1e12))
                            for k, v in mixed["prof"].items()}

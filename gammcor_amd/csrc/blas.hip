@@ -45,7 +45,10 @@ int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B,
     // the whole product is formed even where only the lower triangle is asked
     // for: rocBLAS's SYRK/GEMMT run at a fraction of its GEMM rate on gfx950
     const double flops = 2.0 * g.m * g.n * (double)g.k * g.batch;
-    ProfScope ps(s, sizeof(T) == 8 ? "blas_gemm_f64" : "blas_gemm_f32", flops);
+    // one timing name per transpose pair: each pair runs a kernel of its own
+    static const char* names[2][4] = {{"blas_gemm_f32_nn", "blas_gemm_f32_nt", "blas_gemm_f32_tn", "blas_gemm_f32_tt"},
+                                      {"blas_gemm_f64_nn", "blas_gemm_f64_nt", "blas_gemm_f64_tn", "blas_gemm_f64_tt"}};
+    ProfScope ps(s, names[sizeof(T) == 8][2 * g.ta + g.tb], flops);
     if constexpr (sizeof(T) == 8)
         rs = rocblas_dgemm_strided_batched(h, ta, tb, g.m, g.n, g.k, &alpha, A, (rocblas_int)g.lda, g.sa, B,
                                            (rocblas_int)g.ldb, g.sb, &beta, C, (rocblas_int)g.ldc, g.sc, g.batch);

@@ -3,6 +3,8 @@
     python -m gammcor_amd.profsum stats <dir>            # kernel_stats.csv / kernel_trace.csv
     python -m gammcor_amd.profsum pmc <dir> [--calib-first 3 --calib-mib 1024]
                                                         # counter_collection.csv, KiB per dispatch
+    python -m gammcor_amd.profsum traffic <fetch dir> <write dir>
+                                                        # calibrated bytes/launch per timing name
     python -m gammcor_amd.profsum copy [--mib M] [--reps R]
                                                         # device copy of known size (calibration run)
 
@@ -79,6 +81,26 @@ def pmc_per_kernel(d, calib_first=0):
     return path, res, cal
 
 
+# rocprof kernel name -> the library's timing name (gmc_prof_get)
+_KEYS = (("sytrd_panel_kernel<float", "sytrd_panel_f32"), ("sytrd_panel_kernel<double", "sytrd_panel_f64"),
+         ("gemm_kernel<double", "gemm_f64"), ("gemm_kernel<float", "gemm_f32"),
+         ("sandwich_kernel", "twono1_sandwich"), ("ab_main_kernel", "ab_cas_main"))
+_TENSILE_OPS = {"Ailk_Bljk": "nn", "Ailk_Bjlk": "nt", "Alik_Bljk": "tn", "Alik_Bjlk": "tt"}
+
+
+def timing_key(kernel):
+    """The gmc_prof name of a rocprof kernel name, or None."""
+    for sub, key in _KEYS:
+        if sub in kernel:
+            return key
+    if kernel.startswith("Cijk_"):
+        prec = "f64" if "_DB_" in kernel else "f32" if "_SB_" in kernel else None
+        for sub, ab in _TENSILE_OPS.items():
+            if sub in kernel and prec:
+                return f"blas_gemm_{prec}_{ab}"
+    return None
+
+
 def copy_run(mib, reps):
     import torch
     n = mib * (1 << 20) // 4
@@ -106,6 +128,11 @@ def main(argv=None):
                    help="the first N copy-kernel dispatches are calibration copies of --calib-mib MiB")
     b.add_argument("--calib-mib", type=int, default=1024)
     b.add_argument("--top", type=int, default=12)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--calib-first", type=int, default=3)
+    t.add_argument("--calib-mib", type=int, default=1024)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
     c.add_argument("--reps", type=int, default=5)
@@ -113,6 +140,29 @@ def main(argv=None):
 
     if args.cmd == "copy":
         copy_run(args.mib, args.reps)
+        return
+    if args.cmd == "traffic":
+        # calibrated HBM bytes per launch, FETCH_SIZE + WRITE_SIZE, per timing name
+        per = {}
+        for ctr_dir in (args.fetch_dir, args.write_dir):
+            _, res, cal = pmc_per_kernel(ctr_dir, args.calib_first)
+            for ctr, ks in res.items():
+                vals = cal.get(ctr, [])
+                if not vals or min(vals) <= 0:
+                    raise SystemExit(f"no calibration copies for {ctr} in {ctr_dir}")
+                f = args.calib_mib * 1024.0 / (sum(vals) / len(vals))
+                print(f"# {ctr}: calibration factor {f:.4f} ({len(vals)} copies of {args.calib_mib} MiB)")
+                for k, v in ks.items():
+                    key = timing_key(k)
+                    if key:
+                        d = per.setdefault(key, {})
+                        d.setdefault(ctr, []).extend(x * f * 1024.0 for x in v)
+        print("# TRAFFIC <timing name> <fetch bytes/launch> <write bytes/launch> <launches>")
+        for key in sorted(per):
+            d = per[key]
+            fe = d.get("FETCH_SIZE", [0.0])
+            wr = d.get("WRITE_SIZE", [0.0])
+            print(f"TRAFFIC {key} {sum(fe) / len(fe):.6e} {sum(wr) / len(wr):.6e} {len(fe)}")
         return
     if args.cmd == "stats":
         path, rows = kernel_stats(args.dir)
