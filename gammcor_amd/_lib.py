@@ -22,6 +22,8 @@ SIGNATURES = {
     "gmc_twono1_dev": [_vp, _vp, _c_int, _c_i64, _vp],
     "gmc_acecorr_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp,
                         _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "gmc_acecorr_fofo_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
+                             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_syevd_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
     "gmc_prof_enable": [_c_int],
     "gmc_prof_reset": [],

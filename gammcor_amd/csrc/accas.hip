@@ -18,16 +18,31 @@
 
 namespace gmc {
 
-__device__ inline double tw(const double* t, int p, int q, int r, int s) { return t[pidx(pidx(p, q), pidx(r, s))]; }
-
 struct DevSys {
     int N, nI, nA, nO, nd;
-    const double* twono;
+    const double* twono;     // packed NO integrals, or nullptr for the J/K classes
     const double* occ;
     const int* igem;
     const double* gam;
     const int* indn;
+    const double* J;         // (k l|i j), k, l occupied (fofo.hpp)
+    const double* K;         // (f o|f' o'), o, o' occupied
 };
+
+// (pq|rs) from the packed integrals, or from the J/K classes: two occupied
+// indices in one pair -> J, one occupied index in each pair -> K.  Other
+// integrals are never needed with a non-zero factor; they read as 0.
+__device__ inline double tw(const DevSys& s, int a, int b, int c, int d)
+{
+    if (s.twono) return s.twono[pidx(pidx(a, b), pidx(c, d))];
+    const int nO = s.nO, N = s.N;
+    if (a < nO && b < nO) return s.J[(size_t)(a + nO * b) * N * N + c + (size_t)N * d];
+    if (c < nO && d < nO) return s.J[(size_t)(c + nO * d) * N * N + a + (size_t)N * b];
+    int f1, o1, f2, o2;
+    if (b < nO) { o1 = b; f1 = a; } else if (a < nO) { o1 = a; f1 = b; } else return 0.0;
+    if (d < nO) { o2 = d; f2 = c; } else if (c < nO) { o2 = c; f2 = d; } else return 0.0;
+    return s.K[(size_t)(f1 + (size_t)N * o1) * N * nO + f2 + (size_t)N * o2];
+}
 
 __device__ inline bool same4(const int* g, int i, int j, int k, int l)
 {
@@ -84,14 +99,14 @@ __global__ void ab_aux_kernel(DevSys s, const double* hno, double* h0, double* h
     } else {
         double aux = 0.0;
         for (int t = 0; t < s.N; t++)
-            if (g[t] != g[p]) aux += s.occ[t] * (2.0 * tw(s.twono, t, t, p, q) - tw(s.twono, t, p, t, q));
+            if (g[t] != g[p]) aux += s.occ[t] * (2.0 * tw(s, t, t, p, q) - tw(s, t, p, t, q));
         h0[idx] = h + aux;
         h1[idx] = h;
     }
     // AuxI, AuxIO (interpa.f:3843-3858)
     double a0 = 0, a1 = 0, o0 = 0, o1 = 0;
     for (int t = 0; t < s.nO; t++) {
-        const double v = s.occ[t] * (2.0 * tw(s.twono, p, q, t, t) - tw(s.twono, p, t, q, t));
+        const double v = s.occ[t] * (2.0 * tw(s, p, q, t, t) - tw(s, p, t, q, t));
         const bool same = g[t] == g[p] && g[p] == g[q];
         a1 += v;
         if (same) a0 += v;
@@ -118,7 +133,7 @@ __global__ __launch_bounds__(256) void ab_wmat_kernel(DevSys s, double* w0, doub
     double a0 = 0, a1 = 0;
     for (int x = lane; x < tot; x += 64) {
         const int t = x / (nO * nO), w = (x / nO) % nO, u = x % nO;
-        const double v = tw(s.twono, t, w, p, u) * frdm2(s, w, u, t, r) + tw(s.twono, t, u, p, w) * frdm2(s, w, u, r, t);
+        const double v = tw(s, t, w, p, u) * frdm2(s, w, u, t, r) + tw(s, t, u, p, w) * frdm2(s, w, u, r, t);
         a1 += v;
         if (same4(s.igem, t, w, p, u)) a0 += v;
     }
@@ -143,7 +158,7 @@ __global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hn
     for (int64_t x = threadIdx.x; x < tot; x += 256) {
         const int p = (int)(x / ((int64_t)nO * nO * nO)), q = (int)((x / (nO * nO)) % nO), r = (int)((x / nO) % nO),
                   ss = (int)(x % nO);
-        e += frdm2(s, p, q, r, ss) * tw(s.twono, p, r, q, ss);
+        e += frdm2(s, p, q, r, ss) * tw(s, p, r, q, ss);
     }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
@@ -173,7 +188,7 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
         a1 += (occ[iq] - occ[ir]) * x.h1[ipr];
     }
     const double f0pqrs = same4(g, ip, iq, ir, is) ? 1.0 : 0.0;
-    const double xpqrs = 2.0 * tw(s.twono, ip, iq, ir, is) - tw(s.twono, ip, ir, iq, is);
+    const double xpqrs = 2.0 * tw(s, ip, iq, ir, is) - tw(s, ip, ir, iq, is);
     const int nI = s.nI, nA = s.nA;
     // T1+T2
     if (occ[ip] * occ[ir] != 0.0) {
@@ -190,8 +205,8 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             for (int tt = 0; tt < nA; tt++)
                 for (int uu = 0; uu < nA; uu++) {
                     const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s.twono, is, iq, it, iu) * frdm2(s, ip, iu, ir, it) +
-                           tw(s.twono, is, iu, it, iq) * frdm2(s, ip, iu, it, ir);
+                    sum += tw(s, is, iq, it, iu) * frdm2(s, ip, iu, ir, it) +
+                           tw(s, is, iu, it, iq) * frdm2(s, ip, iu, it, ir);
                 }
             if (same4(g, is, iq, nI, nI)) a0 += sum;
             a1 += sum;
@@ -216,8 +231,8 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             for (int tt = 0; tt < nA; tt++)
                 for (int uu = 0; uu < nA; uu++) {
                     const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s.twono, iu, it, ip, ir) * frdm2(s, is, it, iq, iu) +
-                           tw(s.twono, iu, ir, ip, it) * frdm2(s, is, it, iu, iq);
+                    sum += tw(s, iu, it, ip, ir) * frdm2(s, is, it, iq, iu) +
+                           tw(s, iu, ir, ip, it) * frdm2(s, is, it, iu, iq);
                 }
             if (same4(g, nI, nI, ip, ir)) a0 += sum;
             a1 += sum;
@@ -238,7 +253,7 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             for (int tt = 0; tt < nA; tt++)
                 for (int uu = 0; uu < nA; uu++) {
                     const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s.twono, ip, it, is, iu) * frdm2(s, it, iu, iq, ir);
+                    sum += tw(s, ip, it, is, iu) * frdm2(s, it, iu, iq, ir);
                 }
             if (same4(g, ip, nI, is, nI)) a0 -= sum;
             a1 -= sum;
@@ -255,7 +270,7 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             for (int tt = 0; tt < nA; tt++)
                 for (int uu = 0; uu < nA; uu++) {
                     const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s.twono, it, iq, iu, ir) * frdm2(s, is, ip, iu, it);
+                    sum += tw(s, it, iq, iu, ir) * frdm2(s, is, ip, iu, it);
                 }
             if (same4(g, nI, iq, nI, ir)) a0 -= sum;
             a1 -= sum;
@@ -320,7 +335,7 @@ __global__ __launch_bounds__(256) void ab_main_kernel(DevSys s, AbAux x, double*
 
 static DevSys devsys(const AcSystem& a)
 {
-    return DevSys{a.N, a.nI, a.nA, a.nI + a.nA, a.ndimx, a.twono, a.occ, a.igem, a.gam, a.indn};
+    return DevSys{a.N, a.nI, a.nA, a.nI + a.nA, a.ndimx, a.twono, a.occ, a.igem, a.gam, a.indn, a.J, a.K};
 }
 
 size_t ab_scratch_bytes(const AcSystem& s)
@@ -392,7 +407,7 @@ __global__ void gw_kernel(DevSys s, const double* c, double* Gw)
     const int* g = s.igem;
     double v = 0.0;
     if (!(g[p] == g[r] && g[q] == g[ss] && g[p] == g[q]))
-        v = 2.0 * (c[ss] + c[q]) * (c[p] + c[r]) * tw(s.twono, p, r, q, ss);
+        v = 2.0 * (c[ss] + c[q]) * (c[p] + c[r]) * tw(s, p, r, q, ss);
     Gw[I + (size_t)J * s.nd] = v;
 }
 
@@ -405,7 +420,7 @@ __global__ __launch_bounds__(256) void wdiag_kernel(DevSys s, double* out)
         const int p = s.indn[2 * I] - 1, r = s.indn[2 * I + 1] - 1;
         if (!(g[p] == g[r])) {
             const double np = s.occ[p], nr = s.occ[r];
-            e -= (np * (1.0 - nr) + nr * (1.0 - np)) * tw(s.twono, p, r, p, r);
+            e -= (np * (1.0 - nr) + nr * (1.0 - np)) * tw(s, p, r, p, r);
         }
     }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);

@@ -15,6 +15,9 @@ struct AcSystem {
     const double* ure = nullptr;     // N x N, URe; nullptr means identity
     const double* gam = nullptr;     // nA^4 dense active Gamma(p,q,r,s)
     const int* indn = nullptr;       // 2 x ndimx, IndN(1,i) at [2i], IndN(2,i) at [2i+1], 1-based
+    // with twono == nullptr the integrals come from the two FOFO classes (fofo.hpp)
+    const double* J = nullptr;       // (k l|i j), N^2 nO^2
+    const double* K = nullptr;       // (f o|f' o'), N^2 nO^2
 };
 
 // A+-B at alpha = 0 and alpha = 1 (the matrices are affine in alpha),

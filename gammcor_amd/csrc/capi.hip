@@ -12,6 +12,7 @@
 #include "gemm.hpp"
 #include "transform.hpp"
 #include "accas.hpp"
+#include "fofo.hpp"
 
 namespace gmc {
 
@@ -287,6 +288,45 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
     state().sys_valid = false;   // acws is overwritten: drop the host-call system cache
     AcRun r;
     GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    std::vector<double> w(nalpha);
+    GMC_TRY(ac_alpha_batches(s, r, prec, alpha_batch > 0 ? alpha_batch : 16, nalpha, alphas, w.data(), eig, nneg));
+    double e = 0.0;
+    for (int i = 0; i < nalpha; i++) e += weights[i] * w[i];   // acfd.f:199
+    if (walpha)
+        for (int i = 0; i < nalpha; i++) walpha[i] = w[i];
+    if (ecorr) *ecorr = e;
+    if (ecasscf) *ecasscf = r.ab.ecasscf;
+    return 0;
+}
+
+int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* ao,
+                         const double* uaomo, const double* occ, const int* igem, const double* xone,
+                         const double* rdm2act, int ndimx, const int* indn, int nalpha, const double* alphas,
+                         const double* weights, double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg,
+                         void* stream)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR FOFO: bad sizes");
+    GMC_CHECK(ninact + nact >= 1, "ACECORR FOFO: no occupied orbitals");
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR FOFO: unknown precision mode %d", prec);
+    State& st = state();
+    hipStream_t s = stream ? (hipStream_t)stream : st.stream;
+    st.sys_valid = false;   // acws is overwritten: drop the host-call system cache
+    FofoPlan fp;
+    GMC_TRY(fofo_plan(nbasis, ninact + nact, &fp));
+    GMC_TRY(st.fofo.reserve(carve_bytes({fp.j_elems * 8, fp.k_elems * 8})));
+    Arena ar{static_cast<char*>(st.fofo.ptr), st.fofo.bytes};
+    double* J = ar.take<double>(fp.j_elems);
+    double* K = ar.take<double>(fp.k_elems);
+    GMC_CHECK(ar.fits(), "ACECORR FOFO: layout overflow");
+    GMC_TRY(st.scratch.reserve(fp.scratch_bytes));
+    GMC_TRY(fofo_build(s, fp, ao, uaomo, J, K, st.scratch.ptr));
+    AcRun r;
+    GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, nullptr, occ, igem, xone, nullptr, rdm2act, indn, r));
+    r.sys.J = J;
+    r.sys.K = K;
     GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
     GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
     std::vector<double> w(nalpha);

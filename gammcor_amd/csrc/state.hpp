@@ -21,6 +21,7 @@ struct State {
     Workspace erpaws;      // per-alpha ERPA matrices
     Workspace sysio;       // small arrays of the system cached for host-pointer calls
     Workspace io;          // host <-> device staging of host-pointer calls
+    Workspace fofo;        // the (FF|OO) and (FO|FO) integral classes of the FOFO variant
     const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
     int64_t resident_n = 0;
     double resident_probe[64] = {0};
@@ -42,12 +43,14 @@ struct State {
         if (erpaws.ptr) hipFree(erpaws.ptr);
         if (sysio.ptr) hipFree(sysio.ptr);
         if (io.ptr) hipFree(io.ptr);
+        if (fofo.ptr) hipFree(fofo.ptr);
         scratch = Workspace{};
         resident = Workspace{};
         acws = Workspace{};
         erpaws = Workspace{};
         sysio = Workspace{};
         io = Workspace{};
+        fofo = Workspace{};
         resident_key = nullptr;
         resident_n = 0;
         sys_valid = false;

@@ -141,6 +141,30 @@ def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False, stream=None,
     return out
 
 
+def ACECORR_FOFO(ao, uaomo, dsys, alphas, weights, prec=GMC_FP64, alpha_batch=0, return_eig=False, stream=None):
+    """The FOFO variant (ACEInteg_FOFO, abfofo.f90:3664): packed AO integrals
+    ``ao`` and orbitals ``uaomo`` (URe(NO,AO), column-major, both cuda
+    tensors); only (FF|OO) and (FO|FO) are built.  ``dsys`` supplies occ,
+    IGem, XOne (NO basis), the active 2-RDM and IndN."""
+    import torch
+    al = np.ascontiguousarray(alphas, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    na = len(al)
+    ec, es = np.zeros(1), np.zeros(1)
+    wa = np.zeros(na)
+    nneg = np.zeros(na, dtype=np.int32)
+    eig = torch.zeros(na, dsys.ndimx, dtype=torch.float64, device=dsys.occ.device) if return_eig else None
+    _lib.call("gmc_acecorr_fofo_dev", prec, alpha_batch, dsys.nbasis, dsys.ninact, dsys.nact, ao.data_ptr(),
+              uaomo.data_ptr(), dsys.occ.data_ptr(), dsys.igem.data_ptr(), dsys.xone.data_ptr(),
+              dsys.rdm2act.data_ptr(), dsys.ndimx, dsys.indn.data_ptr(), na, al.ctypes.data, wt.ctypes.data,
+              ec.ctypes.data, wa.ctypes.data, es.ctypes.data, eig.data_ptr() if eig is not None else None,
+              nneg.ctypes.data, stream)
+    out = dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
+    if return_eig:
+        out["eig"] = eig
+    return out
+
+
 # ---------------------------------------------------------------- host drop-ins
 # numpy (host) arrays in and out, the reference's argument meaning; COMMON
 # inputs (NInAcCAS, NAcCAS, IGem) and the rdm2.dat contents are explicit.

@@ -123,6 +123,20 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
                     int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights,
                     double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg, void* stream);
 
+/* The FOFO variant of ACECORR (ACEInteg_FOFO, abfofo.f90:3664-3789, with
+ * AB_CAS_FOFO 8-293 and ACEneERPA_FOFO 3791-3927): the integrals enter as
+ * packed AO integrals ao (NAddr3 order, device) with the AO->NO orbitals
+ * uaomo (URe(NO,AO), N x N, device), and only the two classes the path needs
+ * are built, (FF|OO) and (FO|FO) (tran4_gen, tran.f90:313-423), as batched
+ * half transformations: 2 N^2 nO^2 doubles instead of the N^4/8 packed NO
+ * integrals and the T x T transform scratch.  xone is in the NO basis, the
+ * other arguments and outputs are those of gmc_acecorr_dev. */
+int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* ao,
+                         const double* uaomo, const double* occ, const int* igem, const double* xone,
+                         const double* rdm2act, int ndimx, const int* indn, int nalpha, const double* alphas,
+                         const double* weights, double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg,
+                         void* stream);
+
 /* Building block exported for tests: batched symmetric eigensolver.  A holds
  * batch column-major n x n symmetric matrices (stride n*n) and is overwritten
  * by the eigenvectors (columns); w receives batch x n ascending eigenvalues.
