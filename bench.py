@@ -119,7 +119,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="n240_cas88")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--skip", default="", help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in")
+    ap.add_argument("--skip", default="",
+                    help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in,fofo")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run 3 plain device copies of this size (reference dispatches for "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, see gammcor_amd.profsum)")
@@ -182,6 +183,17 @@ def main():
         results["fp64_batched"] = timed(host.GMC_FP64, 0, 1, 1)
     if "fp64_drop_in" not in args.skip:
         results["fp64_drop_in"] = timed(host.GMC_FP64, 1, 1, 1)
+    if "fofo" not in args.skip:
+        # the FOFO variant: (FF|OO) and (FO|FO) built from the AO integrals in
+        # place of the full transform, same mixed ERPA
+        def fofo_solve():
+            return host.ACECORR_FOFO(ao, ure, dsys, alphas, weights, prec=host.GMC_MIXED, stream=stream)
+        fofo_solve()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rf = fofo_solve()
+        torch.cuda.synchronize()
+        results["fofo_mixed"] = dict(sec=time.perf_counter() - t0, ecorr=rf["ecorr"])
     mixed = results["mixed"]
     fp64 = results.get("fp64_drop_in") or results.get("fp64_batched")
     ref64 = results.get("fp64_batched") or fp64
@@ -233,6 +245,9 @@ def main():
         "fp64_batched_solves_per_s": (1.0 / results["fp64_batched"]["sec"]) if "fp64_batched" in results else None,
         "speedup_mixed_vs_fp64_drop_in": (results["fp64_drop_in"]["sec"] / mixed["sec"]) if "fp64_drop_in" in results
         else None,
+        "fofo_mixed_solves_per_s": (1.0 / results["fofo_mixed"]["sec"]) if "fofo_mixed" in results else None,
+        "fofo_mixed_abs_decorr": abs(results["fofo_mixed"]["ecorr"] - ref64["ecorr"]) if ("fofo_mixed" in results
+                                                                                        and ref64) else None,
         "ecorr_fp64": ref64["ecorr"] if ref64 else None,
         "ecorr_mixed": mixed["ecorr"],
         "abs_decorr": abs(mixed["ecorr"] - ref64["ecorr"]) if ref64 else None,
