@@ -59,3 +59,31 @@ def test_syevd(cuda, prec, kind, n, batch):
         orth = np.linalg.norm(vecs[b].T @ vecs[b] - np.eye(n))
         assert res < 200 * n * eps, res
         assert orth < 200 * n * eps, orth
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_syevd_large_order_global_v(cuda, prec):
+    """n large enough that the panel kernel keeps the Householder vector in
+    global memory instead of LDS (2 n sizeof(T) > 160 KiB for FP64 at
+    n = 10500; FP32 stays in LDS).  Checked on the GPU with torch's FP64
+    products (independent of this library)."""
+    import torch
+    from gammcor_amd import _lib
+    n = 10500
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device=cuda).manual_seed(5)
+    a = torch.randn(n, n, dtype=torch.float64, device=cuda, generator=g)
+    a = 0.5 * (a + a.T)
+    da = a.to(dt).clone()                 # .to() returns `a` itself for float64; the solver overwrites da
+    dw = torch.zeros(1, n, device=cuda, dtype=dt)
+    print(f"syevd {prec} n={n} batch=1 (global v)", flush=True)
+    _lib.call("gmc_syevd_dev", 0 if prec == "f64" else 1, n, 1, da.data_ptr(), dw.data_ptr(), None)
+    v = da.double().T.contiguous()        # row-major view of the column-major result: v[:, k] = vector k
+    w = dw.double()[0]
+    eps = 2.2e-16 if prec == "f64" else 1.2e-7
+    scale = float(w.abs().max())
+    res = torch.linalg.norm(a.to(v.dtype) @ v - v * w[None, :]) / scale
+    orth = torch.linalg.norm(v.T @ v - torch.eye(n, dtype=v.dtype, device=cuda))
+    assert float(res) < 200 * n * eps, float(res)
+    assert float(orth) < 200 * n * eps, float(orth)
+    assert bool(torch.all(w[1:] >= w[:-1]))
