@@ -158,11 +158,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T* A = g.A + (size_t)p * n * n;
     T* W = g.W + (size_t)p * n * nb;
     T* X = g.X + (size_t)p * n * 3 * nb;
-    T* part = g.part + (size_t)p * (1 + G * (2 + 2 * nb));
+    T* part = g.part + (size_t)p * (2 + G * (2 + 2 * nb));
     T* p_alpha = part;
-    T* p_norm = part + 1;
-    T* p_dot = part + 1 + G;
-    T* p_wv = part + 1 + G + 2 * nb * G;
+    T* p_ydot = part + 1;                 // dot part of y at row j+1 (its owner)
+    T* p_norm = part + 2;
+    T* p_dot = part + 2 + G;
+    T* p_vy = part + 2 + G + 2 * nb * G;  // v^T y partials
     unsigned* bar = g.bar + (size_t)p * G * BAR_STRIDE;
     int* err = g.err;
 
@@ -175,22 +176,22 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T* a2 = Aj + nb;
     T* t12 = a2 + nb;                               // 2 nb
     T* red = t12 + 2 * nb;                          // SY_WAVES
-    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal
+    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal [2]=W(j+1, i) of the last column
 
+    // Two barriers per column.  What the next column needs from other
+    // workgroups' rows is formed redundantly after the second barrier: the
+    // coefficient a2 from w^T v = tau (v^T y - 2 t1.t2) with v^T y summed
+    // from per-workgroup partials, and row j+1 of W from the published
+    // partial sums; A(j+1, k+i) is v(j+1).
     unsigned epoch = g.epoch0;
-    T tau_prev = 0;
     for (int i = 0; i < kb; i++) {
         const int j = k + i;
-        if (i > 0) {
-            // w.v of the previous column: its G partials loaded in parallel
-            const T s = block_sum(tid < G ? p_wv[tid] : T(0), red);
-            if (tid == 0) a2[i - 1] = T(-0.5) * tau_prev * s;
-        }
-        __syncthreads();
         for (int c = tid; c < i; c += SY_THREADS) {
-            T ajc = A[j + (size_t)(k + c) * n];
+            // row j of the panel: the last column's entries from this
+            // workgroup's own copies (v still holds the last column's v)
+            const T ajc = c == i - 1 ? v[j] : A[j + (size_t)(k + c) * n];
             Aj[c] = ajc;
-            Wj[c] = W[j + (size_t)c * n] + a2[c] * ajc;
+            Wj[c] = (c == i - 1 ? sc[2] : W[j + (size_t)c * n]) + a2[c] * ajc;
         }
         __syncthreads();
         // step 1: update column j with the panel so far; norms.  The i-term
@@ -417,6 +418,18 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             }
         }
         __syncthreads();
+        {
+            // v^T y partial of this workgroup: own rows' dot parts and its scatter partial
+            T vy = 0;
+            for (int q = tid; q < nslots; q += SY_THREADS) {
+                const int r = owned_row(q, gi, G);
+                if (r > j && r < n) vy += v[r] * y[q];
+                if (r == j + 1 && r < n) *p_ydot = y[q];
+            }
+            for (int c = j + 1 + tid; c < n; c += SY_THREADS) vy += v[c] * u[c];
+            vy = block_sum(vy, red);
+            if (tid == 0) p_vy[gi] = vy;
+        }
         for (int c = j + 1 + tid; c < n; c += SY_THREADS) Pu[(size_t)gi * n + c] = u[c];
         for (int c = wave; c < 2 * i; c += SY_WAVES) {
             const int cc = c < i ? c : c - i;
@@ -471,25 +484,35 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             }
             __syncthreads();
         }
-        T wv = 0;
         for (int q = tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
             if (r <= j || r >= n) continue;
-            T wr = tau * y[q];
-            W[r + (size_t)i * n] = wr;
-            wv += wr * v[r];
+            W[r + (size_t)i * n] = tau * y[q];
             A[r + (size_t)j * n] = v[r];
         }
-        wv = block_sum(wv, red);
-        if (tid == 0) p_wv[gi] = wv;
-        tau_prev = tau;
-        group_barrier(bar, gi, G, epoch, err);
+        // a2[i] = -tau/2 w^T v and W(j+1, i), identical in every workgroup
+        {
+            // one term per thread: the G partials and the i panel terms
+            T x = tid < G ? p_vy[tid] : T(0);
+            if (tid < i) x -= T(2) * t12[tid] * t12[i + tid];
+            const T wv_over_tau = block_sum(x, red);          // v^T y - 2 t1.t2
+            T wnext = 0;
+            if (j + 1 < n) {
+                const int r = j + 1;
+                T yr = tid < G ? Pu[(size_t)tid * n + r] : T(0);
+                if (tid < i) {
+                    const T arc = A[r + (size_t)(k + tid) * n];
+                    yr -= arc * t12[tid] + (W[r + (size_t)tid * n] + a2[tid] * arc) * t12[i + tid];
+                }
+                wnext = tau * (block_sum(yr, red) + *p_ydot);
+            }
+            if (tid == 0) {
+                a2[i] = T(-0.5) * tau * (tau * wv_over_tau);
+                sc[2] = wnext;
+            }
+        }
+        __syncthreads();
     }
-    {
-        const T s = block_sum(tid < G ? p_wv[tid] : T(0), red);
-        if (tid == 0) a2[kb - 1] = T(-0.5) * tau_prev * s;
-    }
-    __syncthreads();
     // X = [V | W | V] for the trailing rows owned here
     const int k1 = k + kb;
     for (int q = tid; q < nslots; q += SY_THREADS) {
@@ -1061,7 +1084,7 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->d, B * n * sizeof(T));
     take(p->e, B * n * sizeof(T));
     take(p->tau, B * n * sizeof(T));
-    take(p->part, B * (1 + G * (2 + 2 * nb)) * sizeof(T));
+    take(p->part, B * (2 + G * (2 + 2 * nb)) * sizeof(T));
     take(p->Pu, B * G * n * sizeof(T));
     take(p->vg, B * G * n * sizeof(T));
     take(p->Q, B * nn * sizeof(T));
@@ -1211,7 +1234,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
                 ProfScope ps(s, sizeof(T) == 8 ? "sytrd_panel_f64" : "sytrd_panel_f32", bytes * B);
                 GMC_TRY(launch_persistent(kfn, dim3(B * G), dim3(SY_THREADS), args, lds, s));
             }
-            epoch += 3 * kb;
+            epoch += 2 * kb;   // two group barriers per column
             const int k1 = k + kb;
             if (k1 < n) {
                 const int mt = n - k1;
