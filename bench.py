@@ -48,6 +48,10 @@ def build_inputs(name, device):
 
 def run_solve(ao, work, ure, dsys, alphas, weights, prec, stream, alpha_batch=0, return_eig=False):
     from gammcor_amd import host, _lib
+    if work is None:
+        # FOFO variant: (FF|OO) and (FO|FO) from the AO integrals, no full transform
+        return host.ACECORR_FOFO(ao, ure, dsys, alphas, weights, prec=prec, stream=stream, alpha_batch=alpha_batch,
+                                 return_eig=return_eig)
     work.copy_(ao)
     _lib.call("gmc_twono1_dev", work.data_ptr(), ure.data_ptr(), dsys.nbasis, work.numel(), stream)
     return host.ACECORR(dsys, alphas, weights, prec=prec, stream=stream, alpha_batch=alpha_batch,
@@ -121,6 +125,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip", default="",
                     help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in,fofo")
+    ap.add_argument("--path", choices=("auto", "incore", "fofo"), default="auto",
+                    help="integrals: incore = TwoNO1 then the packed NO integrals (the reference's ACECORR), "
+                         "fofo = only (FF|OO) and (FO|FO) (its FOFO variant); auto = fofo from NBasis 320 on, "
+                         "where the in-core transform scratch and the ERPA batch no longer fit together")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run 3 plain device copies of this size (reference dispatches for "
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, see gammcor_amd.profsum)")
@@ -141,9 +149,11 @@ def main():
     n = s.nbasis
     nalpha = cfg["nalpha"]
     alphas, weights = host.GauLeg(0.0, 1.0, nalpha)
-    work = torch.empty_like(ao)
+    path = args.path if args.path != "auto" else ("fofo" if n >= 320 else "incore")
+    work = torch.empty_like(ao) if path == "incore" else None
     ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(dev)
-    dsys = host.DeviceSystem(work, s.occ, s.xone, s.gamma_act, n, s.ninact, s.nact, s.nele, device=dev)
+    dsys = host.DeviceSystem(work if work is not None else torch.zeros(1, dtype=torch.float64, device=dev), s.occ,
+                             s.xone, s.gamma_act, n, s.ninact, s.nact, s.nele, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
 
     # peaks of one MI355X (datasheet): HBM3E 8 TB/s; dense FP64 MFMA 78.6 TF/s,
@@ -152,6 +162,7 @@ def main():
     KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
                "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
                "twono1_sandwich": ("mfma", PEAK["mfma_f64"]), "ab_cas_main": ("hbm", PEAK["hbm"])}
+    PHASES = [f"eig_{ph}_{t}" for ph in ("tridiag", "dc", "backtr") for t in ("f32", "f64")]
     for ab in ("nn", "nt", "tn", "tt"):     # rocBLAS products, one kernel per transpose pair
         KERNELS[f"blas_gemm_f64_{ab}"] = ("mfma", PEAK["mfma_f64"])
         KERNELS[f"blas_gemm_f32_{ab}"] = ("mfma", PEAK["mfma_f32"])
@@ -170,20 +181,25 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         prof = {}
+        phases = {}
         if profile:
             _lib.call("gmc_prof_enable", 0)
             for name in KERNELS:
                 ms, cnt, wk = _lib.prof_get(name)
                 if cnt:
                     prof[name] = dict(ms=ms, launches=cnt, work=wk)
-        return dict(sec=dt, ecorr=res["ecorr"], walpha=list(res["walpha"]), eig=res["eig"], prof=prof)
+            for name in PHASES:
+                ms, cnt, wk = _lib.prof_get(name)
+                if cnt:
+                    phases[name] = ms / steps
+        return dict(sec=dt, ecorr=res["ecorr"], walpha=list(res["walpha"]), eig=res["eig"], prof=prof, phases=phases)
 
     results = {"mixed": timed(host.GMC_MIXED, 0, args.steps, args.warmup, profile=True)}
     if "fp64_batched" not in args.skip:
-        results["fp64_batched"] = timed(host.GMC_FP64, 0, 1, 1)
+        results["fp64_batched"] = timed(host.GMC_FP64, 0, 1, 1, profile=True)
     if "fp64_drop_in" not in args.skip:
         results["fp64_drop_in"] = timed(host.GMC_FP64, 1, 1, 1)
-    if "fofo" not in args.skip:
+    if "fofo" not in args.skip and path == "incore":
         # the FOFO variant: (FF|OO) and (FO|FO) built from the AO integrals in
         # place of the full transform, same mixed ERPA
         def fofo_solve():
@@ -240,7 +256,7 @@ def main():
         "data": "synthetic (seeded Cholesky-factor AO integrals, QR orbitals, random-CI RDMs)",
         "config": {"workload": f"AC-CAS {args.config}: NBasis={n}, {s.ninact} inactive, CAS({s.nelact}e,{s.nact}o), "
                                f"NDimX={dsys.ndimx}, {nalpha} Gauss-Legendre alpha points batched, rank-{cfg['rank']} AO",
-                   "nbasis": n, "ndimx": dsys.ndimx, "nalpha": nalpha},
+                   "nbasis": n, "ndimx": dsys.ndimx, "nalpha": nalpha, "integrals": path},
         "fp64_drop_in_solves_per_s": (1.0 / results["fp64_drop_in"]["sec"]) if "fp64_drop_in" in results else None,
         "fp64_batched_solves_per_s": (1.0 / results["fp64_batched"]["sec"]) if "fp64_batched" in results else None,
         "speedup_mixed_vs_fp64_drop_in": (results["fp64_drop_in"]["sec"] / mixed["sec"]) if "fp64_drop_in" in results
@@ -252,6 +268,7 @@ def main():
         "ecorr_mixed": mixed["ecorr"],
         "abs_decorr": abs(mixed["ecorr"] - ref64["ecorr"]) if ref64 else None,
         "max_abs_domega": deig,
+        "eigensolver_ms_per_step": {k: round(v, 1) for r in results.values() for k, v in r.get("phases", {}).items()},
         "roofline": roof,
         "cpu_baseline": None,
     }

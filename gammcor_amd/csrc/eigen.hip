@@ -1207,6 +1207,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     GMC_HIP(hipGetLastError());
 
     // ---------------- tridiagonalisation
+    const bool f64 = sizeof(T) == 8;
+    ProfScope ps_tri(s, f64 ? "eig_tridiag_f64" : "eig_tridiag_f32", 0.0);
     if (n > 1) {
         const int nblk = (n + 63) / 64;
         const int nslots_max = 64 * ((nblk + G - 1) / G);
@@ -1247,6 +1249,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     hipLaunchKernelGGL(sytrd_finish_kernel<T>, dim3((B + 255) / 256), dim3(256), 0, s, A, p.d, n, B);
     hipLaunchKernelGGL(clear_upper_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, A, n);
     GMC_HIP(hipGetLastError());
+    ps_tri.close();
+    ProfScope ps_dc(s, f64 ? "eig_dc_f64" : "eig_dc_f32", 0.0);
 
     // ---------------- divide & conquer
     DcBufs<T> b{p.e, w, p.Q, p.QK, p.QN, p.UB, p.dsort, p.zsort, p.dK, p.zK, p.taus, p.zh, p.EV, p.rotc, p.rots,
@@ -1271,7 +1275,9 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         GMC_HIP(hipGetLastError());
     }
 
+    ps_dc.close();
     // ---------------- back transformation  Z <- H(0)...H(n-2) Z,  Z = Q
+    ProfScope ps_bt(s, f64 ? "eig_backtr_f64" : "eig_backtr_f32", 0.0);
     const int nbb = p.nbb;
     T* Gm = p.Tf;
     T* Tm = p.Tf + (size_t)B * nbb * nbb;
@@ -1289,13 +1295,13 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             GMC_HIP(hipGetLastError());
             GemmShape g2{kb, n, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
                          (int64_t)nn, B, 0};
-            GMC_TRY(gemm<T>(s, g2, T(1), V, Zs, T(0), p.QK));
+            GMC_TRY(gemm_big<T>(s, g2, T(1), V, Zs, T(0), p.QK));
             GemmShape g3{kb, n, kb, false, false, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb * nbb,
                          (int64_t)nn, (int64_t)nn, B, 0};
             GMC_TRY(gemm<T>(s, g3, T(1), Tm, p.QK, T(0), p.QN));
             GemmShape g4{m, n, kb, false, false, (int64_t)n, (int64_t)nbb, (int64_t)n, (int64_t)nn, (int64_t)nn,
                          (int64_t)nn, B, 0};
-            GMC_TRY(gemm<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
+            GMC_TRY(gemm_big<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
         }
     }
     hipLaunchKernelGGL(copy_kernel<T>, dim3(grid1((int64_t)B * nn)), dim3(256), 0, s, p.Q, A, (int64_t)B * nn);

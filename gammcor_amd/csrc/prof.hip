@@ -46,11 +46,12 @@ ProfScope::ProfScope(hipStream_t st, const char* name, double work) : s(st)
     slot = (int)g_recs.size() - 1;
 }
 
-ProfScope::~ProfScope()
+void ProfScope::close()
 {
     if (slot < 0) return;
     std::lock_guard<std::mutex> lk(g_mu);
     (void)hipEventRecord(g_recs[slot].b, s);
+    slot = -1;
 }
 
 }  // namespace gmc

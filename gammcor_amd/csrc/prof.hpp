@@ -11,7 +11,8 @@ struct ProfScope {
     int slot = -1;
     hipStream_t s = nullptr;
     ProfScope(hipStream_t s, const char* name, double work);
-    ~ProfScope();
+    ~ProfScope() { close(); }
+    void close();   // record the end now (idempotent)
 };
 
 bool prof_enabled();
