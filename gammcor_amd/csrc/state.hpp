@@ -37,13 +37,8 @@ struct State {
         eig32.clear();
         blas_release(blas);
         blas = nullptr;
-        if (scratch.ptr) hipFree(scratch.ptr);
-        if (resident.ptr) hipFree(resident.ptr);
-        if (acws.ptr) hipFree(acws.ptr);
-        if (erpaws.ptr) hipFree(erpaws.ptr);
-        if (sysio.ptr) hipFree(sysio.ptr);
-        if (io.ptr) hipFree(io.ptr);
-        if (fofo.ptr) hipFree(fofo.ptr);
+        for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo})
+            if (w->ptr) (void)hipFree(w->ptr);
         scratch = Workspace{};
         resident = Workspace{};
         acws = Workspace{};

@@ -6,7 +6,7 @@ namespace gmc {
 struct TransformPlan {
     int N = 0;
     int64_t T = 0;
-    int NP = 0, BR = 0, nbands = 0;
+    int chunk = 0;              // pair columns per batch of the half transforms
     size_t scratch_bytes = 0;   // device scratch needed by twono1_device
 };
 

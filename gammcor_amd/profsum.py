@@ -84,7 +84,7 @@ def pmc_per_kernel(d, calib_first=0):
 # rocprof kernel name -> the library's timing name (gmc_prof_get)
 _KEYS = (("sytrd_panel_kernel<float", "sytrd_panel_f32"), ("sytrd_panel_kernel<double", "sytrd_panel_f64"),
          ("gemm_kernel<double", "gemm_f64"), ("gemm_kernel<float", "gemm_f32"),
-         ("sandwich_kernel", "twono1_sandwich"), ("ab_main_kernel", "ab_cas_main"))
+         ("ab_main_kernel", "ab_cas_main"))
 _TENSILE_OPS = {"Ailk_Bljk": "nn", "Ailk_Bjlk": "nt", "Alik_Bljk": "tn", "Alik_Bjlk": "tt"}
 
 

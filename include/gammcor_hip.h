@@ -149,7 +149,7 @@ int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
  * (ms), the launch count and the algorithmic work (bytes for memory-bound
  * kernels, flops for MFMA kernels).  Names: "sytrd_panel_f64",
  * "sytrd_panel_f32", "gemm_f64", "gemm_f32", "blas_gemm_f64_<ab>", "blas_gemm_f32_<ab>" (ab = nn, nt,
- * tn, tt: the transposes of the rocBLAS product), "twono1_sandwich",
+ * tn, tt: the transposes of the rocBLAS product), "twono1" (the whole transform),
  * "dc_merge_gemm_f64", "dc_merge_gemm_f32", "ab_cas_main". */
 int gmc_prof_enable(int on);
 int gmc_prof_reset(void);
