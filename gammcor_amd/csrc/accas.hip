@@ -467,16 +467,28 @@ __global__ void sqrt_scale_kernel(T* V, const T* lam, int n, int* nneg)
     if (i == 0 && l < T(0)) atomicAdd(&nneg[b], 1);
 }
 
+// M_ji <- M_ij for i > j: the strict upper triangle from the lower one, in
+// 64 x 64 tiles through LDS so reads and writes are both contiguous.  Only
+// tiles on or below the diagonal are read and only the transposed tile is
+// written, so the workgroups do not overlap.
 template <class T>
-__global__ void symmetrize_kernel(T* M, int n)
+__global__ __launch_bounds__(256) void mirror_lower_kernel(T* M, int n)
 {
-    const int b = blockIdx.z;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n || i <= j) return;
-    T* m = M + (size_t)b * n * n;
-    const T v = T(0.5) * (m[i + (size_t)j * n] + m[j + (size_t)i * n]);
-    m[i + (size_t)j * n] = v;
-    m[j + (size_t)i * n] = v;
+    __shared__ T tile[64][65];
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (ti < tj) return;
+    T* m = M + (size_t)blockIdx.z * n * n;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int i0 = ti * 64, j0 = tj * 64;
+    for (int y = ty; y < 64; y += 4) {
+        const int i = i0 + tx, j = j0 + y;
+        if (i < n && j < n) tile[y][tx] = m[i + (size_t)j * n];
+    }
+    __syncthreads();
+    for (int y = ty; y < 64; y += 4) {
+        const int i = i0 + y, j = j0 + tx;
+        if (i < n && j < n && i > j) m[j + (size_t)i * n] = tile[tx][y];
+    }
 }
 
 // Normalisation of interpa.f:1748-1776 (2/omega Y^T A- Y = 2 omega for Y = S z)
@@ -589,8 +601,11 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     // M = S A- S
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm_big<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
-    GMC_TRY(gemm_big<T>(st, gN, T(1), Sb, Mb, T(0), Tb));
-    hipLaunchKernelGGL(symmetrize_kernel<T>, gsq, blk, 0, st, Tb, n);
+    GemmShape gNl = gN;
+    gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
+    GMC_TRY(gemm_big<T>(st, gNl, T(1), Sb, Mb, T(0), Tb));
+    const int nt64 = (n + 63) / 64;
+    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, B), blk, 0, st, Tb, n);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<T>(st, ep, Tb, om2));
     // Y = S Z
@@ -956,8 +971,11 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
-    GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, Sb, 0.0, Ap));
-    hipLaunchKernelGGL(symmetrize_kernel<double>, gsq, blk, 0, st, Ap, n);
+    GemmShape gNl = gN;
+    gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
+    GMC_TRY(gemm_big<double>(st, gNl, 1.0, AX, Sb, 0.0, Ap));
+    const int nt64 = (n + 63) / 64;
+    hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, Ap, n);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());

@@ -31,10 +31,8 @@ void blas_release(void* h)
 }
 
 template <class T>
-int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C)
+static int gemm_one(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C)
 {
-    static const bool own = getenv("GMC_OWN_GEMM") != nullptr;   // diagnostic: the library's own kernel
-    if (own) return gemm<T>(s, g, alpha, A, B, beta, C);
     if (g.m <= 0 || g.n <= 0 || g.batch <= 0) return 0;
     rocblas_handle h = nullptr;
     GMC_TRY(blas_handle(&h));
@@ -42,8 +40,6 @@ int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B,
     const rocblas_operation ta = g.ta ? rocblas_operation_transpose : rocblas_operation_none;
     const rocblas_operation tb = g.tb ? rocblas_operation_transpose : rocblas_operation_none;
     rocblas_status rs;
-    // the whole product is formed even where only the lower triangle is asked
-    // for: rocBLAS's SYRK/GEMMT run at a fraction of its GEMM rate on gfx950
     const double flops = 2.0 * g.m * g.n * (double)g.k * g.batch;
     // one timing name per transpose pair: each pair runs a kernel of its own
     static const char* names[2][4] = {{"blas_gemm_f32_nn", "blas_gemm_f32_nt", "blas_gemm_f32_tn", "blas_gemm_f32_tt"},
@@ -56,6 +52,44 @@ int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B,
         rs = rocblas_sgemm_strided_batched(h, ta, tb, g.m, g.n, g.k, &alpha, A, (rocblas_int)g.lda, g.sa, B,
                                            (rocblas_int)g.ldb, g.sb, &beta, C, (rocblas_int)g.ldc, g.sc, g.batch);
     GMC_CHECK(rs == rocblas_status_success, "rocBLAS gemm failed with status %d", (int)rs);
+    return 0;
+}
+
+// Block rows of the lower-triangle form (at least 2).  rocBLAS's SYRK and
+// GEMMT run at a fraction of its GEMM rate on gfx950, so a lower triangle is
+// formed as P plain products, block row p covering rows [r_p, r_p+1) and
+// columns [0, r_p+1): (P + 1) / 2P of the full product's flops.
+static int tri_parts()
+{
+    static const int parts = [] {
+        const char* e = getenv("GMC_TRI_PARTS");
+        const int v = e ? atoi(e) : 8;
+        return v < 2 ? 2 : (v > 32 ? 32 : v);
+    }();
+    return parts;
+}
+
+template <class T>
+int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C)
+{
+    static const bool own = getenv("GMC_OWN_GEMM") != nullptr;   // diagnostic: the library's own kernel
+    if (own) return gemm<T>(s, g, alpha, A, B, beta, C);
+    constexpr int TRI_MIN = 1024;   // below this a single full product is as fast
+    if (g.uplo != 1 || g.m != g.n || g.m < TRI_MIN) return gemm_one<T>(s, g, alpha, A, B, beta, C);
+    const int P = tri_parts();
+    int r0 = 0;
+    for (int p = 1; p <= P; p++) {
+        // boundaries on multiples of 16 rows, the last one at m
+        const int r1 = p == P ? g.m : (int)(((int64_t)g.m * p / P) & ~int64_t(15));
+        if (r1 <= r0) continue;
+        GemmShape gp = g;
+        gp.m = r1 - r0;
+        gp.n = r1;
+        gp.uplo = 0;
+        const T* Ap = g.ta ? A + (size_t)r0 * g.lda : A + r0;   // rows [r0, r1) of op(A)
+        GMC_TRY(gemm_one<T>(s, gp, alpha, Ap, B, beta, C + r0));
+        r0 = r1;
+    }
     return 0;
 }
 

@@ -20,8 +20,9 @@ struct GemmShape {
 template <class T>
 int gemm(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
 
-// Large plain products (rocBLAS, blas.hip).  The whole of C is written even
-// when g.uplo asks for the lower triangle only.
+// Large plain products (rocBLAS, blas.hip).  With g.uplo == 1 (square C) the
+// lower triangle is formed in block rows; the strict upper triangle outside
+// the diagonal blocks is left as it was.
 template <class T>
 int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
 void blas_release(void* handle);
