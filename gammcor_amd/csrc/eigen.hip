@@ -111,13 +111,31 @@ struct PanelArgs {
     int vec;   // 16-byte SYMV path: n a multiple of 16 / sizeof(T) and A 16-byte aligned
 };
 
-// Rows owned by workgroup gi: 64-row blocks gi, gi+G, ...; slot q -> row.
-__device__ inline int owned_row(int q, int gi, int G) { return 64 * (gi + G * (q >> 6)) + (q & 63); }
-// First slot of the first owned 64-row block that reaches row `row`.
+// Rows owned by workgroup gi: one unit of SY_OB rows per round of G units,
+// taken in snake order (round t owns unit tG + gi for even t, tG + G-1-gi for
+// odd t).  The lower-triangle SYMV costs n - r per row r and the workgroups
+// wait for the slowest at every group barrier, so the units are small: at
+// column j the unit the sweep is inside is the only part of a round that is
+// owned unevenly, which bounds the imbalance by SY_OB (n - j) elements.
+// Slot q -> row; rows grow with the slot and row >= q.
+constexpr int SY_OB = 8;
+__device__ inline int owned_unit(int t, int gi, int G) { return t * G + ((t & 1) ? G - 1 - gi : gi); }
+__device__ inline int owned_row(int q, int gi, int G) { return SY_OB * owned_unit(q / SY_OB, gi, G) + q % SY_OB; }
+// Number of slots (SY_OB per owned unit) of workgroup gi over nunit units.
+__device__ inline int owned_slots(int nunit, int gi, int G)
+{
+    const int T = nunit / G, rem = nunit - T * G;
+    const int off = (T & 1) ? G - 1 - gi : gi;
+    return SY_OB * (T + (off < rem ? 1 : 0));
+}
+// First slot of the first owned unit that reaches row `row`.
 __device__ inline int first_slot(int row, int gi, int G)
 {
-    const int t = row - 63 - 64 * gi;
-    return t <= 0 ? 0 : 64 * ((t + 64 * G - 1) / (64 * G));
+    // unit(t) lies in [tG, tG + G - 1]: start a round early and step forward
+    int t = (row - (SY_OB - 1)) / (SY_OB * G) - 1;
+    if (t < 0) t = 0;
+    while (SY_OB * owned_unit(t, gi, G) + SY_OB - 1 < row) t++;
+    return SY_OB * t;
 }
 
 constexpr int SY_THREADS = 1024, SY_WAVES = SY_THREADS / 64;
@@ -152,8 +170,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
     const int p = blockIdx.x / G, gi = blockIdx.x % G;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nblk = (n + 63) / 64;
-    const int nslots = 64 * ((nblk - gi + G - 1) / G);
+    const int nslots = owned_slots((n + SY_OB - 1) / SY_OB, gi, G);
 
     T* A = g.A + (size_t)p * n * n;
     T* W = g.W + (size_t)p * n * nb;
@@ -268,7 +285,11 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             // reads rows c..c+VN-1 (c = chunk + VN*l) of each, UV chunks in
             // flight.  The first chunk (diagonal block) is masked, the rest
             // are not: every row there lies below all KV columns.
-            for (int qg = wave; KV * qg < nslots; qg += SY_WAVES) {
+            static_assert(SY_OB % KV == 0, "a column group must lie inside one unit");
+            // groups in snake order over the waves: group work falls with the row
+            for (int t = 0;; t++) {
+                const int qg = t * SY_WAVES + ((t & 1) ? SY_WAVES - 1 - wave : wave);
+                if (KV * qg >= nslots) break;
                 const int r0 = owned_row(KV * qg, gi, G);
                 if (r0 + KV - 1 <= j || r0 >= n) continue;
                 T vk[KV], s[KV];
@@ -357,9 +378,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                 }
             }
         } else
-        // each wave takes KC consecutive owned columns r0..r0+KC-1 (a 64-block
-        // holds 64/KC such groups): one LDS atomic per row for their scatters
-        for (int qg = wave; KC * qg < nslots; qg += SY_WAVES) {
+        // each wave takes KC consecutive owned columns r0..r0+KC-1 (a unit
+        // holds SY_OB/KC such groups): one LDS atomic per row for their scatters
+        for (int t = 0;; t++) {
+            static_assert(SY_OB % KC == 0, "a column group must lie inside one unit");
+            const int qg = t * SY_WAVES + ((t & 1) ? SY_WAVES - 1 - wave : wave);
+            if (KC * qg >= nslots) break;
             const int r0 = owned_row(KC * qg, gi, G);
             if (r0 + KC - 1 <= j || r0 >= n) continue;
             bool ok[KC];
@@ -1210,8 +1234,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     const bool f64 = sizeof(T) == 8;
     ProfScope ps_tri(s, f64 ? "eig_tridiag_f64" : "eig_tridiag_f32", 0.0);
     if (n > 1) {
-        const int nblk = (n + 63) / 64;
-        const int nslots_max = 64 * ((nblk + G - 1) / G);
+        const int nunit = (n + SY_OB - 1) / SY_OB;
+        const int nslots_max = SY_OB * ((nunit + G - 1) / G);
         const size_t small = (size_t)(nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
         const bool vlds = (2 * (size_t)n * sizeof(T) + small) <= 160 * 1024;
         const size_t lds = (vlds ? 2 : 1) * (size_t)n * sizeof(T) + small;
