@@ -157,7 +157,7 @@ struct alignas(16) Vec16 {
 template <class T>
 __device__ inline Vec16<T> ld16(const T* p) { return *reinterpret_cast<const Vec16<T>*>(p); }
 
-template <class T, bool VLDS, int KV, int UV>
+template <class T, bool VLDS, bool PLDS, int KV, int UV>
 __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 {
     // columns per wave in the SYMV: FP32 halves the HBM time per element, so
@@ -170,10 +170,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
     const int p = blockIdx.x / G, gi = blockIdx.x % G;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nslots = owned_slots((n + SY_OB - 1) / SY_OB, gi, G);
+    const int nunit = (n + SY_OB - 1) / SY_OB;
+    const int nslots = owned_slots(nunit, gi, G);
+    const int nsm = SY_OB * ((nunit + G - 1) / G);   // slots of the largest workgroup
 
     T* A = g.A + (size_t)p * n * n;
-    T* W = g.W + (size_t)p * n * nb;
+    T* W = g.W + (size_t)p * n * nb;   // W'(r, c) = W(r, c) + a2(c) V(r, c), the corrected panel
     T* X = g.X + (size_t)p * n * 3 * nb;
     T* part = g.part + (size_t)p * (2 + G * (2 + 2 * nb));
     T* p_alpha = part;
@@ -186,100 +188,103 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 
     T* u = reinterpret_cast<T*>(smem);             // n
     T* v = VLDS ? u + n : g.vg + ((size_t)p * G + gi) * n;
-    T* y = VLDS ? u + 2 * n : u + n;                // nslots
+    T* y = VLDS ? u + 2 * n : u + n;                // nsm
     T* Pu = g.Pu + (size_t)p * G * n;
-    T* Wj = y + nslots;                             // nb
-    T* Aj = Wj + nb;
-    T* a2 = Aj + nb;
+    T* Vp = y + nsm;                                // PLDS: V(r, c) of the owned rows, nb x nsm
+    T* Wp = Vp + (PLDS ? (size_t)nb * nsm : 0);     // PLDS: W'(r, c) of the owned rows
+    T* Ar = Wp + (PLDS ? (size_t)nb * nsm : 0);     // V(j+1, c), nb
+    T* Wr = Ar + nb;                                // W'(j+1, c), nb
+    T* a2 = Wr + nb;
     T* t12 = a2 + nb;                               // 2 nb
     T* red = t12 + 2 * nb;                          // SY_WAVES
-    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal [2]=W(j+1, i) of the last column
+    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal
 
-    // Two barriers per column.  What the next column needs from other
-    // workgroups' rows is formed redundantly after the second barrier: the
-    // coefficient a2 from w^T v = tau (v^T y - 2 t1.t2) with v^T y summed
-    // from per-workgroup partials, and row j+1 of W from the published
-    // partial sums; A(j+1, k+i) is v(j+1).
-    unsigned epoch = g.epoch0;
-    for (int i = 0; i < kb; i++) {
-        const int j = k + i;
-        for (int c = tid; c < i; c += SY_THREADS) {
-            // row j of the panel: the last column's entries from this
-            // workgroup's own copies (v still holds the last column's v)
-            const T ajc = c == i - 1 ? v[j] : A[j + (size_t)(k + c) * n];
-            Aj[c] = ajc;
-            Wj[c] = (c == i - 1 ? sc[2] : W[j + (size_t)c * n]) + a2[c] * ajc;
-        }
-        __syncthreads();
-        // step 1: update column j with the panel so far; norms.  The i-term
-        // sum of a row is split over P threads and gathered in LDS (u is free
-        // here), so its loads are issued side by side.
-        const int qa = first_slot(j, gi, G), na = nslots - qa;
-        if (i > 0 && na > 0) {
-            // slot q - qa <= row < n for every row below n, so u[0..n) suffices
-            for (int q = tid; q < na; q += SY_THREADS)
-                if (owned_row(qa + q, gi, G) < n) u[q] = T(0);
-            __syncthreads();
-            const int P = max(1, min(i, SY_THREADS / na));
-            for (int idx = tid; idx < na * P; idx += SY_THREADS) {
-                const int qq = idx % na, part = idx / na;
-                const int r = owned_row(qa + qq, gi, G);
-                if (r < j || r >= n) continue;
-                T acc = 0;
-                for (int c = part; c < i; c += P) {
-                    const T arc = A[r + (size_t)(k + c) * n];
-                    acc += arc * Wj[c] + (W[r + (size_t)c * n] + a2[c] * arc) * Aj[c];
-                }
-                lds_add(&u[qq], acc);
-            }
-            __syncthreads();
-        }
+    // panel entries of an owned row (slot q, row r): LDS copies or the global panel
+    auto vpan = [&](int c, int q, int r) -> T {
+        if constexpr (PLDS) return Vp[(size_t)c * nsm + q];
+        else return A[r + (size_t)(k + c) * n];
+    };
+    auto wpan = [&](int c, int q, int r) -> T {
+        if constexpr (PLDS) return Wp[(size_t)c * nsm + q];
+        else return W[r + (size_t)c * n];
+    };
+
+    // Column k arrives updated by the trailing product: its diagonal, alpha
+    // and norm partial.  Every later column is updated and normed at the end
+    // of the column before it.
+    {
         T nrm = 0;
-        for (int q = qa + tid; q < nslots; q += SY_THREADS) {
+        for (int q = first_slot(k, gi, G) + tid; q < nslots; q += SY_THREADS) {
             const int r = owned_row(q, gi, G);
-            if (r < j || r >= n) continue;
-            T a = A[r + (size_t)j * n];
-            if (i > 0) {
-                a -= u[q - qa];
-                A[r + (size_t)j * n] = a;
-            }
-            if (r == j) g.d[(size_t)p * n + j] = a;
-            if (r == j + 1) *p_alpha = a;
-            if (r >= j + 2) nrm += a * a;
+            if (r < k || r >= n) continue;
+            const T a = A[r + (size_t)k * n];
+            if (r == k) g.d[(size_t)p * n + k] = a;
+            if (r == k + 1) *p_alpha = a;
+            if (r >= k + 2) nrm += a * a;
         }
         nrm = block_sum(nrm, red);
         if (tid == 0) p_norm[gi] = nrm;
+    }
+
+    // Two group barriers per column:
+    //   1. the column's norm partials and its updated entries are published:
+    //      reflector, v, SYMV, panel dots and the v^T y partial;
+    //   2. the SYMV partials are published: every workgroup forms the panel
+    //      coefficients t1, t2, a2 and row j+1 of W' redundantly, then its own
+    //      rows of y, W' and V, and the update and norm of column j+1.
+    unsigned epoch = g.epoch0;
+    for (int i = 0; i < kb; i++) {
+        const int j = k + i;
         group_barrier(bar, gi, G, epoch, err);
 
-        // step 2: reflector, v into LDS; step 3: SYMV and panel dots
-        const T xn2 = block_sum(tid < G ? p_norm[tid] : T(0), red);
-        if (tid == 0) {
-            const T alpha = *p_alpha;
-            T tau, beta, scal;
-            if (xn2 == T(0)) {
-                tau = 0;
-                beta = alpha;
-                scal = 0;
-            } else {
-                beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-                tau = (beta - alpha) / beta;
-                scal = T(1) / (alpha - beta);
+        // reflector: the norm partials and the column are loaded together,
+        // the column scaled once tau is known
+        {
+            const T xpart = tid < G ? p_norm[tid] : T(0);
+            for (int c0 = j + 2 + tid; c0 < n; c0 += 8 * SY_THREADS) {
+                T tmp[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int c = c0 + e * SY_THREADS;
+                    tmp[e] = c < n ? A[c + (size_t)j * n] : T(0);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int c = c0 + e * SY_THREADS;
+                    if (c < n) v[c] = tmp[e];
+                }
             }
-            sc[0] = tau;
-            sc[1] = scal;
-            if (gi == 0) {
-                g.tau[(size_t)p * n + j] = tau;
-                g.e[(size_t)p * n + j] = beta;
+            const T xn2 = block_sum(xpart, red);
+            if (tid == 0) {
+                const T alpha = *p_alpha;
+                T tau, beta, scal;
+                if (xn2 == T(0)) {
+                    tau = 0;
+                    beta = alpha;
+                    scal = 0;
+                } else {
+                    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                    tau = (beta - alpha) / beta;
+                    scal = T(1) / (alpha - beta);
+                }
+                sc[0] = tau;
+                sc[1] = scal;
+                if (gi == 0) {
+                    g.tau[(size_t)p * n + j] = tau;
+                    g.e[(size_t)p * n + j] = beta;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
         const T tau = sc[0], scal = sc[1];
-        for (int c = j + 1 + tid; c < n; c += SY_THREADS) v[c] = (c == j + 1) ? T(1) : scal * A[c + (size_t)j * n];
+        for (int c = j + 1 + tid; c < n; c += SY_THREADS) {
+            v[c] = c == j + 1 ? T(1) : scal * v[c];
+            u[c] = T(0);
+        }
+        for (int c = tid; c < 2 * i; c += SY_THREADS) t12[c] = T(0);
         __syncthreads();
-        // SYMV on the lower triangle: one wave per owned column r, four loads
-        // in flight per lane; dot part -> y, scatter part -> u (LDS atomics)
-        for (int c = j + 1 + tid; c < n; c += SY_THREADS) u[c] = T(0);
-        __syncthreads();
+
+        // SYMV on the lower triangle: dot part -> y, scatter part -> u (LDS atomics)
         if (g.vec) {
             // 16-byte path: a wave takes KV consecutive owned columns; lane l
             // reads rows c..c+VN-1 (c = chunk + VN*l) of each, UV chunks in
@@ -455,40 +460,67 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             if (tid == 0) p_vy[gi] = vy;
         }
         for (int c = j + 1 + tid; c < n; c += SY_THREADS) Pu[(size_t)gi * n + c] = u[c];
-        for (int c = wave; c < 2 * i; c += SY_WAVES) {
-            const int cc = c < i ? c : c - i;
-            T s = 0;
-            for (int q = lane; q < nslots; q += 64) {
-                const int r = owned_row(q, gi, G);
-                if (r <= j || r >= n) continue;
-                T arc = A[r + (size_t)(k + cc) * n];
-                T x = c < i ? W[r + (size_t)cc * n] + a2[cc] * arc : arc;
-                s += x * v[r];
+        // panel dots over the own rows: c < i with W'(:, c), c >= i with V(:, c - i)
+        {
+            const int qb = first_slot(j + 1, gi, G);
+            for (int c = wave; c < 2 * i; c += SY_WAVES) {
+                const int cc = c < i ? c : c - i;
+                T s = 0;
+                for (int q = qb + lane; q < nslots; q += 64) {
+                    const int r = owned_row(q, gi, G);
+                    if (r <= j || r >= n) continue;
+                    s += (c < i ? wpan(cc, q, r) : vpan(cc, q, r)) * v[r];
+                }
+                s = wave_sum(s);
+                if (lane == 0) p_dot[gi * 2 * nb + c] = s;
             }
-            s = wave_sum(s);
-            if (lane == 0) p_dot[gi * 2 * nb + c] = s;
         }
         group_barrier(bar, gi, G, epoch, err);
 
-        // step 4: w = tau (y - V t1 - W t2), w.v partial, scaled v into A.
-        // t12[c] = sum over the G partials, split over PP threads per c.
-        if (i > 0) {
-            const int nc = 2 * i;
-            for (int c = tid; c < nc; c += SY_THREADS) t12[c] = T(0);
-            __syncthreads();
-            const int PP = max(1, min(G, SY_THREADS / nc));
-            for (int idx = tid; idx < nc * PP; idx += SY_THREADS) {
-                const int c = idx % nc, part = idx / nc;
-                T acc = 0;
-                for (int q = part; q < G; q += PP) acc += p_dot[q * 2 * nb + c];
-                lds_add(&t12[c], acc);
+        // coefficients shared by all workgroups, from one round of loads:
+        // t12 = [W'^T v; V^T v] summed over the partials, v^T y, the partials of
+        // y(j+1), and row j+1 of the panel
+        const bool more = i + 1 < kb;   // column j+1 belongs to this panel
+        {
+            const T pvy = tid < G ? p_vy[tid] : T(0);
+            const T pyr = tid < G ? Pu[(size_t)tid * n + j + 1] : T(0);
+            const T ydot1 = *p_ydot;
+            if (i > 0) {
+                const int nc = 2 * i;
+                const int PP = max(1, min(G, SY_THREADS / nc));
+                for (int idx = tid; idx < nc * PP; idx += SY_THREADS) {
+                    const int c = idx % nc, part = idx / nc;
+                    T acc = 0;
+                    for (int q = part; q < G; q += PP) acc += p_dot[q * 2 * nb + c];
+                    lds_add(&t12[c], acc);
+                }
+            }
+            for (int c = tid; c < i; c += SY_THREADS) {
+                Ar[c] = A[j + 1 + (size_t)(k + c) * n];
+                Wr[c] = W[j + 1 + (size_t)c * n];
             }
             __syncthreads();
+            // a2(i) = -tau/2 w^T v with w^T v = tau (v^T y - 2 t1.t2), and
+            // W'(j+1, i) = tau y(j+1) + a2(i) v(j+1), v(j+1) = 1
+            T x = pvy, yr = pyr;
+            if (tid < i) {
+                x -= T(2) * t12[tid] * t12[i + tid];
+                yr -= Ar[tid] * t12[tid] + Wr[tid] * t12[i + tid];
+            }
+            const T wv_over_tau = block_sum(x, red);
+            const T ysum = block_sum(yr, red);
+            if (tid == 0) {
+                const T a2i = T(-0.5) * tau * (tau * wv_over_tau);
+                a2[i] = a2i;
+                Ar[i] = T(1);
+                Wr[i] = tau * (ysum + ydot1) + a2i;
+            }
         }
-        // y[q] += (the G workgroups' scatter partials) - (panel terms), each
+        // own rows of y: the G scatter partials and the panel terms, each
         // row's G + i terms split over P threads
+        const int qb = first_slot(j + 1, gi, G), nbq = nslots - qb;
         {
-            const int qb = first_slot(j + 1, gi, G), nbq = nslots - qb, nterm = G + i;
+            const int nterm = G + i;
             const int P = nbq > 0 ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
             for (int idx = tid; idx < nbq * P; idx += SY_THREADS) {
                 const int q = qb + idx % nbq, part = idx / nbq;
@@ -500,51 +532,72 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                         acc += Pu[(size_t)t * n + r];
                     } else {
                         const int c = t - G;
-                        const T arc = A[r + (size_t)(k + c) * n];
-                        acc -= arc * t12[c] + (W[r + (size_t)c * n] + a2[c] * arc) * t12[i + c];
+                        acc -= vpan(c, q, r) * t12[c] + wpan(c, q, r) * t12[i + c];
                     }
                 }
                 lds_add(&y[q], acc);
             }
             __syncthreads();
         }
-        for (int q = tid; q < nslots; q += SY_THREADS) {
-            const int r = owned_row(q, gi, G);
-            if (r <= j || r >= n) continue;
-            W[r + (size_t)i * n] = tau * y[q];
-            A[r + (size_t)j * n] = v[r];
-        }
-        // a2[i] = -tau/2 w^T v and W(j+1, i), identical in every workgroup
+        // W'(r, i) and V(r, i) = v(r) of the own rows; u[0, nbq) cleared for
+        // the update of column j+1 (slot q - qb <= row < n)
         {
-            // one term per thread: the G partials and the i panel terms
-            T x = tid < G ? p_vy[tid] : T(0);
-            if (tid < i) x -= T(2) * t12[tid] * t12[i + tid];
-            const T wv_over_tau = block_sum(x, red);          // v^T y - 2 t1.t2
-            T wnext = 0;
-            if (j + 1 < n) {
-                const int r = j + 1;
-                T yr = tid < G ? Pu[(size_t)tid * n + r] : T(0);
-                if (tid < i) {
-                    const T arc = A[r + (size_t)(k + tid) * n];
-                    yr -= arc * t12[tid] + (W[r + (size_t)tid * n] + a2[tid] * arc) * t12[i + tid];
+            const T a2i = a2[i];
+            for (int q = qb + tid; q < nslots; q += SY_THREADS) {
+                const int r = owned_row(q, gi, G);
+                if (r >= n) continue;
+                u[q - qb] = T(0);
+                if (r <= j) continue;
+                const T vr = v[r], wr = tau * y[q] + a2i * vr;
+                W[r + (size_t)i * n] = wr;
+                A[r + (size_t)j * n] = vr;
+                if constexpr (PLDS) {
+                    Vp[(size_t)i * nsm + q] = vr;
+                    Wp[(size_t)i * nsm + q] = wr;
                 }
-                wnext = tau * (block_sum(yr, red) + *p_ydot);
             }
-            if (tid == 0) {
-                a2[i] = T(-0.5) * tau * (tau * wv_over_tau);
-                sc[2] = wnext;
-            }
+            __syncthreads();
         }
-        __syncthreads();
+        if (!more) break;
+        // column j+1 of the own rows: A(r, j+1) -= sum_{c<=i} V(r,c) W'(j+1,c) + W'(r,c) V(j+1,c),
+        // each row's i+1 terms split over P threads into u
+        {
+            const int nterm = i + 1;
+            const int P = nbq > 0 ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
+            for (int idx = tid; idx < nbq * P; idx += SY_THREADS) {
+                const int qq = idx % nbq, part = idx / nbq;
+                const int q = qb + qq;
+                const int r = owned_row(q, gi, G);
+                if (r <= j || r >= n) continue;
+                T acc = 0;
+                for (int c = part; c < nterm; c += P) acc += vpan(c, q, r) * Wr[c] + wpan(c, q, r) * Ar[c];
+                lds_add(&u[qq], acc);
+            }
+            __syncthreads();
+        }
+        {
+            const int j1 = j + 1;
+            T nrm = 0;
+            for (int q = qb + tid; q < nslots; q += SY_THREADS) {
+                const int r = owned_row(q, gi, G);
+                if (r < j1 || r >= n) continue;
+                const T a = A[r + (size_t)j1 * n] - u[q - qb];
+                A[r + (size_t)j1 * n] = a;
+                if (r == j1) g.d[(size_t)p * n + j1] = a;
+                if (r == j1 + 1) *p_alpha = a;
+                if (r >= j1 + 2) nrm += a * a;
+            }
+            nrm = block_sum(nrm, red);
+            if (tid == 0) p_norm[gi] = nrm;
+        }
     }
-    // X = [V | W | V] for the trailing rows owned here
+    // X = [V | W' | V] for the trailing rows owned here
     const int k1 = k + kb;
     for (int q = tid; q < nslots; q += SY_THREADS) {
         const int r = owned_row(q, gi, G);
         if (r < k1 || r >= n) continue;
         for (int c = 0; c < kb; c++) {
-            T vr = A[r + (size_t)(k + c) * n];
-            T wr = W[r + (size_t)c * n] + a2[c] * vr;
+            const T vr = vpan(c, q, r), wr = wpan(c, q, r);
             X[r + (size_t)c * n] = vr;
             X[r + (size_t)(kb + c) * n] = wr;
             X[r + (size_t)(2 * kb + c) * n] = vr;
@@ -1236,15 +1289,24 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     if (n > 1) {
         const int nunit = (n + SY_OB - 1) / SY_OB;
         const int nslots_max = SY_OB * ((nunit + G - 1) / G);
+        // LDS: u and v (n each; v moves to a global copy when it does not
+        // fit), y and the small arrays, and the own rows of the V and W'
+        // panels when they fit beside them
+        constexpr size_t LDS_MAX = 160 * 1024;
         const size_t small = (size_t)(nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
-        const bool vlds = (2 * (size_t)n * sizeof(T) + small) <= 160 * 1024;
-        const size_t lds = (vlds ? 2 : 1) * (size_t)n * sizeof(T) + small;
-        GMC_CHECK(lds <= 160 * 1024, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
+        const size_t panels = 2 * (size_t)nb * nslots_max * sizeof(T);
+        const size_t vec_n = (size_t)n * sizeof(T);
+        const bool vlds = 2 * vec_n + small <= LDS_MAX;
+        const bool plds = (vlds ? 2 : 1) * vec_n + small + panels <= LDS_MAX;
+        const size_t lds = (vlds ? 2 : 1) * vec_n + small + (plds ? panels : 0);
+        GMC_CHECK(lds <= LDS_MAX, "syevd: n=%d needs %zu bytes of LDS per workgroup", n, lds);
         // 16-byte SYMV shape per type (measured at n=2664/4524, batch 16):
         // FP32 8 columns x 1 chunk, FP64 4 columns x 2 chunks in flight
         constexpr int KV = sizeof(T) == 8 ? 4 : 8, UV = sizeof(T) == 8 ? 2 : 1;
-        const void* kfn = vlds ? (const void*)sytrd_panel_kernel<T, true, KV, UV>
-                               : (const void*)sytrd_panel_kernel<T, false, KV, UV>;
+        const void* kfn = vlds ? (plds ? (const void*)sytrd_panel_kernel<T, true, true, KV, UV>
+                                       : (const void*)sytrd_panel_kernel<T, true, false, KV, UV>)
+                               : (plds ? (const void*)sytrd_panel_kernel<T, false, true, KV, UV>
+                                       : (const void*)sytrd_panel_kernel<T, false, false, KV, UV>);
         GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned epoch = 0;
         for (int k = 0; k < n - 1; k += nb) {
