@@ -629,34 +629,38 @@ __global__ void clear_upper_kernel(T* A, int n)
 
 // ====================================================================== ormtr
 // T factor (upper triangular, ld nbb) of H(k)...H(k+kb-1) = I - V T V^T from
-// the Gram matrix Gm = V^T V (ld nbb) and tau, built in LDS (nbb <= 64).
+// the Gram matrix Gm = V^T V (ld nbb) and tau, built in LDS (dynamic,
+// nbb (nbb + 1) + nbb elements).
+constexpr int BT_MAX = 128;   // largest back-transformation block
 template <class T>
 __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int k, int kb, int nbb)
 {
+    extern __shared__ char smem[];
     const int p = blockIdx.x;
     const T* G = Gm + (size_t)p * nbb * nbb;
     T* Tm = Tf + (size_t)p * nbb * nbb;
     const T* ta = tau + (size_t)p * n + k;
-    __shared__ T t[64][65];
-    __shared__ T tmp[64];
-    for (int idx = threadIdx.x; idx < 64 * 65; idx += 256) (&t[0][0])[idx] = T(0);
+    const int ld = nbb + 1;
+    T* t = reinterpret_cast<T*>(smem);   // t[r * ld + c]
+    T* tmp = t + (size_t)nbb * ld;
+    for (int idx = threadIdx.x; idx < nbb * ld; idx += 256) t[idx] = T(0);
     __syncthreads();
     for (int i = 0; i < kb; i++) {
         const T ti = ta[i];
-        if (threadIdx.x < i) tmp[threadIdx.x] = G[threadIdx.x + (size_t)i * nbb];
+        if ((int)threadIdx.x < i) tmp[threadIdx.x] = G[threadIdx.x + (size_t)i * nbb];
         __syncthreads();
-        if (threadIdx.x < i) {
+        if ((int)threadIdx.x < i) {
             const int c = threadIdx.x;
             T acc = 0;
-            for (int c2 = c; c2 < i; c2++) acc += t[c][c2] * tmp[c2];
-            t[c][i] = -ti * acc;
+            for (int c2 = c; c2 < i; c2++) acc += t[c * ld + c2] * tmp[c2];
+            t[c * ld + i] = -ti * acc;
         }
-        if (threadIdx.x == 0) t[i][i] = ti;
+        if (threadIdx.x == 0) t[i * ld + i] = ti;
         __syncthreads();
     }
     for (int idx = threadIdx.x; idx < nbb * nbb; idx += 256) {
         const int r = idx % nbb, c = idx / nbb;
-        Tm[idx] = (r < 64 && c < 64) ? t[r][c] : T(0);
+        Tm[idx] = t[r * ld + c];
     }
 }
 
@@ -1237,7 +1241,9 @@ int eig_plan(int n, int batch, EigPlan<T>* p)
     GMC_HIP(hipMalloc(&p->d_split, split.size()));
     GMC_HIP(hipMemcpy(p->d_split, split.data(), split.size(), hipMemcpyHostToDevice));
 
-    p->nbb = std::min(64, n);
+    // back-transformation block: the rank-nbb updates of Z read and write Z
+    // once per block, so the block is as wide as the T factor in LDS allows
+    p->nbb = std::min(BT_MAX, n);
     GMC_TRY(eig_layout(p, nullptr));
     return 0;
 }
@@ -1368,6 +1374,9 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     T* Gm = p.Tf;
     T* Tm = p.Tf + (size_t)B * nbb * nbb;
     if (n > 1) {
+        const size_t lds_t = ((size_t)nbb * (nbb + 1) + nbb) * sizeof(T);
+        GMC_HIP(hipFuncSetAttribute((const void*)larft_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_t));
         const int last = ((n - 2) / nbb) * nbb;
         for (int k = last; k >= 0; k -= nbb) {
             const int kb = std::min(nbb, n - 1 - k);
@@ -1377,7 +1386,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             GemmShape g1{kb, kb, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
                          (int64_t)nbb * nbb, B, 0};
             GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gm));
-            hipLaunchKernelGGL(larft_kernel<T>, dim3(B), dim3(256), 0, s, Gm, p.tau, Tm, n, k, kb, nbb);
+            hipLaunchKernelGGL(larft_kernel<T>, dim3(B), dim3(256), lds_t, s, Gm, p.tau, Tm, n, k, kb, nbb);
             GMC_HIP(hipGetLastError());
             GemmShape g2{kb, n, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
                          (int64_t)nn, B, 0};

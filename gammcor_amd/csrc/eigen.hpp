@@ -24,7 +24,7 @@ template <class T>
 struct EigPlan {
     int n = 0, batch = 0;
     int nb = 32;            // tridiagonalisation panel width
-    int nbb = 64;           // back-transformation block width (<= 64)
+    int nbb = 128;          // back-transformation block width (<= 128)
     int G = 1;              // workgroups cooperating on one problem
     std::vector<DcLevel> levels;
     unsigned char* d_split = nullptr;   // split flags (n)
