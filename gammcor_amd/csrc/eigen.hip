@@ -629,8 +629,10 @@ __global__ void clear_upper_kernel(T* A, int n)
 
 // ====================================================================== ormtr
 // T factor (upper triangular, ld nbb) of H(k)...H(k+kb-1) = I - V T V^T from
-// the Gram matrix Gm = V^T V (ld nbb) and tau, built in LDS (dynamic,
-// nbb (nbb + 1) + nbb elements).
+// the Gram matrix Gm = V^T V (ld nbb) and tau: column i of T is
+// -tau_i T(0:i, 0:i) G(0:i, i), diagonal tau_i.  T, the upper triangle of G
+// and tau live in LDS (dynamic, kb (kb + 1) + kb elements): upper triangles
+// packed by columns, column c at c (c + 1) / 2.
 constexpr int BT_MAX = 128;   // largest back-transformation block
 template <class T>
 __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int k, int kb, int nbb)
@@ -639,28 +641,56 @@ __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T
     const int p = blockIdx.x;
     const T* G = Gm + (size_t)p * nbb * nbb;
     T* Tm = Tf + (size_t)p * nbb * nbb;
-    const T* ta = tau + (size_t)p * n + k;
-    const int ld = nbb + 1;
-    T* t = reinterpret_cast<T*>(smem);   // t[r * ld + c]
-    T* tmp = t + (size_t)nbb * ld;
-    for (int idx = threadIdx.x; idx < nbb * ld; idx += 256) t[idx] = T(0);
+    const int tri = kb * (kb + 1) / 2;
+    T* t = reinterpret_cast<T*>(smem);
+    T* gs = t + tri;
+    T* ts = gs + tri;
+    // the upper triangle of G, eight independent loads in flight per thread
+    const int sq = kb * kb;
+    for (int b0 = 0; b0 < sq; b0 += 8 * 256) {
+        T gv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = b0 + e * 256 + threadIdx.x, r = idx % kb, c = idx / kb;
+            gv[e] = (idx < sq && r <= c) ? G[r + (size_t)c * nbb] : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = b0 + e * 256 + threadIdx.x, r = idx % kb, c = idx / kb;
+            if (idx < sq && r <= c) {
+                gs[c * (c + 1) / 2 + r] = gv[e];
+                t[c * (c + 1) / 2 + r] = T(0);
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < kb; i += 256) ts[i] = tau[(size_t)p * n + k + i];
     __syncthreads();
     for (int i = 0; i < kb; i++) {
-        const T ti = ta[i];
-        if ((int)threadIdx.x < i) tmp[threadIdx.x] = G[threadIdx.x + (size_t)i * nbb];
-        __syncthreads();
+        const int ci = i * (i + 1) / 2;
         if ((int)threadIdx.x < i) {
             const int c = threadIdx.x;
-            T acc = 0;
-            for (int c2 = c; c2 < i; c2++) acc += t[c * ld + c2] * tmp[c2];
-            t[c * ld + i] = -ti * acc;
+            // four independent terms per pass keep LDS loads in flight
+            T acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            int c2 = c, at = c * (c + 1) / 2 + c;   // at = position of t(c, c2)
+            for (; c2 + 3 < i; c2 += 4) {
+                acc0 += t[at] * gs[ci + c2];
+                acc1 += t[at + c2 + 1] * gs[ci + c2 + 1];
+                acc2 += t[at + 2 * c2 + 3] * gs[ci + c2 + 2];
+                acc3 += t[at + 3 * c2 + 6] * gs[ci + c2 + 3];
+                at += 4 * c2 + 10;
+            }
+            for (; c2 < i; c2++) {
+                acc0 += t[at] * gs[ci + c2];
+                at += c2 + 1;
+            }
+            t[ci + c] = -ts[i] * ((acc0 + acc1) + (acc2 + acc3));
         }
-        if (threadIdx.x == 0) t[i * ld + i] = ti;
+        if (threadIdx.x == 0) t[ci + i] = ts[i];
         __syncthreads();
     }
     for (int idx = threadIdx.x; idx < nbb * nbb; idx += 256) {
         const int r = idx % nbb, c = idx / nbb;
-        Tm[idx] = t[r * ld + c];
+        Tm[idx] = (c < kb && r <= c) ? t[c * (c + 1) / 2 + r] : T(0);
     }
 }
 
