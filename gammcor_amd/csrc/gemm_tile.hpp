@@ -11,6 +11,10 @@
 // MFMA 16x16x4 operand/result layout (one block, checked on gfx950): lane l
 // supplies A[l%16][l/16] and B[l/16][l%16]; accumulator register r of lane l
 // holds D[l/16 + 4r][l%16] for f64 and D[4(l/16) + r][l%16] for f32.
+// The tile is formed transposed (D = op(B)^T op(A)^T, the B fragment in the
+// A slot): lane l then holds C rows l%16 of one column per register, so each
+// C load and store of a wave covers 16 consecutive rows of the column-major
+// matrix instead of 16 columns.
 #pragma once
 #include "gemm.hpp"
 
@@ -97,8 +101,8 @@ __device__ void gemm_tile(const GemmShape& g, int bi, int bj, T alpha, const T* 
             for (int y = 0; y < 2; y++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int i = bi + wi * 32 + x * 16 + Mfma<T>::row(lane, r);
-                    const int j = bj + wj * 32 + y * 16 + (lane & 15);
+                    const int i = bi + wi * 32 + x * 16 + (lane & 15);
+                    const int j = bj + wj * 32 + y * 16 + Mfma<T>::row(lane, r);
                     cpre[x][y][r] = (i < g.m && j < g.n && (g.uplo == 0 || i >= j)) ? C[i + (int64_t)j * g.ldc]
                                                                                        : T(0);
                 }
@@ -119,7 +123,7 @@ __device__ void gemm_tile(const GemmShape& g, int bi, int bj, T alpha, const T* 
 #pragma unroll
             for (int x = 0; x < 2; x++)
 #pragma unroll
-                for (int y = 0; y < 2; y++) acc[x][y] = Mfma<T>::op(af[x], bf[y], acc[x][y]);
+                for (int y = 0; y < 2; y++) acc[x][y] = Mfma<T>::op(bf[y], af[x], acc[x][y]);
         }
     }
 
@@ -129,8 +133,8 @@ __device__ void gemm_tile(const GemmShape& g, int bi, int bj, T alpha, const T* 
         for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int i = bi + wi * 32 + x * 16 + Mfma<T>::row(lane, r);
-                int j = bj + wj * 32 + y * 16 + (lane & 15);
+                const int i = bi + wi * 32 + x * 16 + (lane & 15);
+                const int j = bj + wj * 32 + y * 16 + Mfma<T>::row(lane, r);
                 if (i < g.m && j < g.n && (g.uplo == 0 || i >= j)) {
                     T v = alpha * acc[x][y][r];
                     if (beta != T(0)) v += beta * cpre[x][y][r];
