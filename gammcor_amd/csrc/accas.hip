@@ -439,20 +439,49 @@ int integrand_weights(hipStream_t st, const AcSystem& a, const double* cicoef, d
 }
 
 // ------------------------------------------------------------ ERPA
-// A(alpha) = (1-alpha) A(0) + alpha A(1), symmetrised, into the batch buffers.
+// A+(alpha_b) and A-(alpha_b) of every alpha point: the symmetric parts of
+// A(0) and A(1) combined affinely (A+-B is affine in alpha; ERPASYMM1
+// symmetrises, interpa.f:1680-1687).  One workgroup per 64 x 64 tile on or
+// below the diagonal: each base tile and its transposed partner are read
+// once and symmetrised in LDS, then every alpha point's tile and its mirror
+// are written along columns.
 template <class T>
-__global__ void form_kernel(const double* p0, const double* p1, const double* m0, const double* m1,
-                            const double* alphas, T* Ap, T* Am, int n)
+__global__ __launch_bounds__(256) void form_kernel(const double* p0, const double* p1, const double* m0,
+                                                   const double* m1, const double* alphas, int nalpha, T* Ap, T* Am,
+                                                   int n)
 {
-    const int b = blockIdx.z;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    const double al = alphas[b];
-    const size_t ij = i + (size_t)j * n, ji = j + (size_t)i * n;
-    const double xp = 0.5 * (((1.0 - al) * p0[ij] + al * p1[ij]) + ((1.0 - al) * p0[ji] + al * p1[ji]));
-    const double xm = 0.5 * (((1.0 - al) * m0[ij] + al * m1[ij]) + ((1.0 - al) * m0[ji] + al * m1[ji]));
-    Ap[(size_t)b * n * n + ij] = (T)xp;
-    Am[(size_t)b * n * n + ij] = (T)xm;
+    __shared__ double s0[64][65], s1[64][65], tmp[64][65];
+    int ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const int i0 = ti * 64, j0 = tj * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // dst[y][x] = sym(X)(i0 + x, j0 + y)
+    auto load_sym = [&](const double* X, double (*dst)[65]) {
+        for (int y = ty; y < 64; y += 4) {
+            const int i = i0 + tx, j = j0 + y;
+            dst[y][tx] = (i < n && j < n) ? X[i + (size_t)j * n] : 0.0;
+            const int i2 = j0 + tx, j2 = i0 + y;
+            tmp[y][tx] = (i2 < n && j2 < n) ? X[i2 + (size_t)j2 * n] : 0.0;   // X(j0 + x, i0 + y)
+        }
+        __syncthreads();
+        for (int y = ty; y < 64; y += 4) dst[y][tx] = 0.5 * (dst[y][tx] + tmp[tx][y]);
+        __syncthreads();
+    };
+    for (int which = 0; which < 2; which++) {
+        load_sym(which ? m0 : p0, s0);
+        load_sym(which ? m1 : p1, s1);
+        T* out = which ? Am : Ap;
+        for (int b = 0; b < nalpha; b++) {
+            const double al = alphas[b];
+            T* o = out + (size_t)b * n * n;
+            for (int y = ty; y < 64; y += 4) {
+                const int i = i0 + tx, j = j0 + y;
+                if (i < n && j < n) o[i + (size_t)j * n] = (T)((1.0 - al) * s0[y][tx] + al * s1[y][tx]);
+                const int i2 = j0 + tx, j2 = i0 + y;
+                if (ti != tj && i2 < n && j2 < n) o[i2 + (size_t)j2 * n] = (T)((1.0 - al) * s0[tx][y] + al * s1[tx][y]);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // V' = V |lambda|^(1/4) column-wise; count negative eigenvalues of A+.
@@ -580,7 +609,9 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
 
     const dim3 gsq((n + 255) / 256, n, B), blk(256);
     // A+(alpha) -> E, A-(alpha) -> Tb
-    hipLaunchKernelGGL(form_kernel<T>, gsq, blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, E, Tb, n);
+    const int nt64f = (n + 63) / 64;
+    hipLaunchKernelGGL(form_kernel<T>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, B,
+                       E, Tb, n);
     GMC_HIP(hipGetLastError());
     if (Am_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: A- output is available from the FP64 solve");
@@ -950,7 +981,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     const dim3 gsq((n + 255) / 256, n, B), blk(256);
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
 
-    hipLaunchKernelGGL(form_kernel<double>, gsq, blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, Ap, Am, n);
+    const int nt64f = (n + 63) / 64;
+    hipLaunchKernelGGL(form_kernel<double>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1,
+                       d_al, B, Ap, Am, n);
     // eig(A+) in FP32, refined in FP64
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());

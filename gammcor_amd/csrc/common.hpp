@@ -84,4 +84,15 @@ __host__ __device__ inline int64_t pidx(int64_t i, int64_t j) {
     return a * (a + 1) / 2 + b;
 }
 
+// Tile t of an enumeration of the lower triangle of a tile grid, row by row
+// (tile (ta, tb) with tb <= ta is number ta (ta + 1) / 2 + tb).
+__device__ inline void lower_tile(int t, int& ta, int& tb)
+{
+    int a = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((a + 1) * (a + 2) / 2 <= t) a++;
+    while (a * (a + 1) / 2 > t) a--;
+    ta = a;
+    tb = t - a * (a + 1) / 2;
+}
+
 }  // namespace gmc

@@ -78,11 +78,6 @@ __global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict
 }
 
 // ---------------------------------------------------------------- half transform
-__device__ inline int64_t dpair(int64_t i, int64_t j)
-{
-    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
-}
-
 int transform_plan(int N, TransformPlan* p)
 {
     p->N = N;
@@ -97,29 +92,61 @@ int transform_plan(int N, TransformPlan* p)
 // Over a batch of columns: gather each column into an N x N matrix,
 // Z = U M U^T as two batched products, scatter the b <= a half of Z into the
 // output (mode 0: column c of H; mode 1: the pairs <= c into row c of the
-// packed result).
-__global__ void col_gather_kernel(const double* __restrict__ S, int64_t T, int64_t c0, int nc, int N,
-                                  double* __restrict__ M)
+// packed result).  Both kernels work on 64 x 64 tiles (a block, b block) of
+// the lower triangle through LDS: the packed pair a(a+1)/2 + b is contiguous
+// in b and the N x N matrix in a, so reads and writes are contiguous on both
+// sides.  blockIdx.x enumerates the lower tiles, blockIdx.y the column.
+__global__ __launch_bounds__(256) void col_gather_kernel(const double* __restrict__ S, int64_t T, int64_t c0, int nc,
+                                                         int N, double* __restrict__ M)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x, n = blockIdx.y, b = blockIdx.z;
-    if (m >= N || b >= nc) return;
-    M[(size_t)b * N * N + m + (size_t)n * N] = S[dpair(m, n) + (c0 + b) * T];
+    __shared__ double tile[64][65];
+    const int bb = blockIdx.y;
+    if (bb >= nc) return;
+    int ta, tb;
+    lower_tile(blockIdx.x, ta, tb);
+    const int a0 = ta * 64, b0 = tb * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const double* col = S + (c0 + bb) * T;
+    double* m = M + (size_t)bb * N * N;
+    // tile[y][x] = V(a0 + y, b0 + x), b <= a, read along b
+    for (int y = ty; y < 64; y += 4) {
+        const int a = a0 + y, b = b0 + tx;
+        tile[y][tx] = (a < N && b <= a) ? col[(int64_t)a * (a + 1) / 2 + b] : 0.0;
+    }
+    __syncthreads();
+    // M(a, b) along a and its mirror M(b, a) along b
+    for (int y = ty; y < 64; y += 4) {
+        const int b = b0 + y, a = a0 + tx;
+        if (a < N && b <= a) m[a + (size_t)b * N] = tile[tx][y];
+        const int a2 = a0 + y, b2 = b0 + tx;
+        if (a2 < N && b2 <= a2) m[b2 + (size_t)a2 * N] = tile[y][tx];
+    }
 }
 
-__global__ void col_scatter_kernel(const double* __restrict__ Z, int64_t T, int64_t c0, int nc, int N, int mode,
-                                   double* __restrict__ out)
+__global__ __launch_bounds__(256) void col_scatter_kernel(const double* __restrict__ Z, int64_t T, int64_t c0, int nc,
+                                                          int N, int mode, double* __restrict__ out)
 {
-    const int64_t pr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // pair index a(a+1)/2 + b
+    __shared__ double tile[64][65];
     const int bb = blockIdx.y;
-    if (pr >= T || bb >= nc) return;
-    int a = (int)((sqrt(8.0 * (double)pr + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(a + 1) * (a + 2) / 2 <= pr) a++;
-    while ((int64_t)a * (a + 1) / 2 > pr) a--;
-    const int b = (int)(pr - (int64_t)a * (a + 1) / 2);
+    if (bb >= nc) return;
+    int ta, tb;
+    lower_tile(blockIdx.x, ta, tb);
+    const int a0 = ta * 64, b0 = tb * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const double* z = Z + (size_t)bb * N * N;
     const int64_t c = c0 + bb;
-    const double z = Z[(size_t)bb * N * N + a + (size_t)b * N];
-    if (mode == 0) out[pr + c * T] = z;
-    else if (pr <= c) out[c * (c + 1) / 2 + pr] = z;
+    // tile[y][x] = Z(a0 + x, b0 + y), read along a
+    for (int y = ty; y < 64; y += 4) {
+        const int a = a0 + tx, b = b0 + y;
+        tile[y][tx] = (a < N && b < N) ? z[a + (size_t)b * N] : 0.0;
+    }
+    __syncthreads();
+    // packed pair a(a+1)/2 + b written along b
+    for (int y = ty; y < 64; y += 4) {
+        const int a = a0 + y, b = b0 + tx;
+        if (a >= N || b > a) continue;
+        const int64_t pr = (int64_t)a * (a + 1) / 2 + b;
+        if (mode == 0) out[pr + c * T] = tile[tx][y];
+        else if (pr <= c) out[c * (c + 1) / 2 + pr] = tile[tx][y];
+    }
 }
 
 static int run_batched(hipStream_t s, const TransformPlan& p, const double* S, const double* U, double* out,
@@ -129,14 +156,14 @@ static int run_batched(hipStream_t s, const TransformPlan& p, const double* S, c
     const int64_t nn = (int64_t)N * N;
     for (int64_t c0 = 0; c0 < p.T; c0 += p.chunk) {
         const int nc = (int)std::min<int64_t>(p.chunk, p.T - c0);
-        hipLaunchKernelGGL(col_gather_kernel, dim3((N + 255) / 256, N, nc), dim3(256), 0, s, S, p.T, c0, nc, N, M);
+        const int nt = (N + 63) / 64, ntl = nt * (nt + 1) / 2;
+        hipLaunchKernelGGL(col_gather_kernel, dim3(ntl, nc), dim3(256), 0, s, S, p.T, c0, nc, N, M);
         GMC_HIP(hipGetLastError());
         GemmShape g1{N, N, N, false, false, N, N, N, 0, nn, nn, nc, 0};     // Y = U M
         GMC_TRY(gemm_big<double>(s, g1, 1.0, U, M, 0.0, Y));
         GemmShape g2{N, N, N, false, true, N, N, N, nn, 0, nn, nc, 0};      // Z = Y U^T
         GMC_TRY(gemm_big<double>(s, g2, 1.0, Y, U, 0.0, M));
-        hipLaunchKernelGGL(col_scatter_kernel, dim3((unsigned)((p.T + 255) / 256), nc), dim3(256), 0, s, M, p.T, c0,
-                           nc, N, mode, out);
+        hipLaunchKernelGGL(col_scatter_kernel, dim3(ntl, nc), dim3(256), 0, s, M, p.T, c0, nc, N, mode, out);
         GMC_HIP(hipGetLastError());
     }
     return 0;
