@@ -87,3 +87,34 @@ def test_syevd_large_order_global_v(cuda, prec):
     assert float(res) < 200 * n * eps, float(res)
     assert float(orth) < 200 * n * eps, float(orth)
     assert bool(torch.all(w[1:] >= w[:-1]))
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_syevd_erpa_batch_shape(cuda, prec):
+    """The ERPA batch shape at N=240: n = 4524, 16 problems, 16 workgroups per
+    problem.  In FP64 the own rows of the V and W' panels do not fit in LDS
+    beside u and v and are read from the global panel; in FP32 they stay in
+    LDS.  Checked with torch's FP64 batched products on the GPU."""
+    import torch
+    from gammcor_amd import _lib
+    n, batch = 4524, 16
+    dt = torch.float64 if prec == "f64" else torch.float32
+    g = torch.Generator(device=cuda).manual_seed(7)
+    a = torch.randn(batch, n, n, dtype=torch.float64, device=cuda, generator=g)
+    a = (0.5 * (a + a.transpose(1, 2))).contiguous()
+    da = a.to(dt).clone()
+    dw = torch.zeros(batch, n, device=cuda, dtype=dt)
+    print(f"syevd {prec} n={n} batch={batch} (ERPA batch shape)", flush=True)
+    _lib.call("gmc_syevd_dev", 0 if prec == "f64" else 1, n, batch, da.data_ptr(), dw.data_ptr(), None)
+    v = da.double().transpose(1, 2).contiguous()   # v[b][:, k] = eigenvector k
+    del da
+    w = dw.double()
+    eps = 2.2e-16 if prec == "f64" else 1.2e-7
+    eye = torch.eye(n, dtype=torch.float64, device=cuda)
+    for b in range(batch):
+        scale = float(w[b].abs().max())
+        res = torch.linalg.norm(a[b] @ v[b] - v[b] * w[b][None, :]) / scale
+        orth = torch.linalg.norm(v[b].T.contiguous() @ v[b] - eye)
+        assert float(res) < 200 * n * eps, (b, float(res))
+        assert float(orth) < 200 * n * eps, (b, float(orth))
+        assert bool(torch.all(w[b, 1:] >= w[b, :-1]))
