@@ -623,8 +623,11 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     hipLaunchKernelGGL(sqrt_scale_kernel<T>, gsq, blk, 0, st, E, lam, n, out.nneg);
     GMC_HIP(hipGetLastError());
     // S = V|L|^(1/2)V^T
-    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    // S is symmetric: its lower triangle, mirrored
+    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
     GMC_TRY(gemm_big<T>(st, gS, T(1), E, E, T(0), Sb));
+    const int nt64 = (n + 63) / 64;
+    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, B), blk, 0, st, Sb, n);
     if (S_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: A+^(1/2) output is available from the FP64 solve");
         GMC_HIP(hipMemcpyAsync(S_out, Sb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -635,7 +638,6 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     GemmShape gNl = gN;
     gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
     GMC_TRY(gemm_big<T>(st, gNl, T(1), Sb, Mb, T(0), Tb));
-    const int nt64 = (n + 63) / 64;
     hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, B), blk, 0, st, Tb, n);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<T>(st, ep, Tb, om2));
@@ -1000,14 +1002,16 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
     hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
     GMC_HIP(hipGetLastError());
-    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    // S is symmetric: its lower triangle, mirrored
+    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
     GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
+    const int nt64 = (n + 63) / 64;
+    hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, AX, n);
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
     GemmShape gNl = gN;
     gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
     GMC_TRY(gemm_big<double>(st, gNl, 1.0, AX, Sb, 0.0, Ap));
-    const int nt64 = (n + 63) / 64;
     hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, Ap, n);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
