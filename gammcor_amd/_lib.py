@@ -34,6 +34,8 @@ SIGNATURES = {
     "gmc_aceneerpa": [_vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int],
     "gmc_aceinteg": [_c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int,
                      _c_double, _vp, _vp, _vp, _vp],
+    "gmc_aceinteg_fofo": [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int,
+                          _c_double, _vp, _vp, _vp, _vp],
     "gmc_acecorr": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
                     _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_forget_resident": [],

@@ -313,7 +313,8 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
     GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR FOFO: unknown precision mode %d", prec);
     State& st = state();
     hipStream_t s = stream ? (hipStream_t)stream : st.stream;
-    st.sys_valid = false;   // acws is overwritten: drop the host-call system cache
+    st.sys_valid = false;   // acws and fofo are overwritten: drop the host-call caches
+    st.fofo_key[0] = st.fofo_key[1] = nullptr;
     FofoPlan fp;
     GMC_TRY(fofo_plan(nbasis, ninact + nact, &fp));
     GMC_TRY(st.fofo.reserve(carve_bytes({fp.j_elems * 8, fp.k_elems * 8})));

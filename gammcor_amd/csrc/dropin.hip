@@ -73,6 +73,41 @@ static int resident_twono(hipStream_t s, const double* host, int64_t ninte2, con
     return 0;
 }
 
+// Device copies of the FOFO integral classes J = (FF|OO) and K = (FO|FO)
+// (host arrays in the record order of the reference's FFOO and FOFO files);
+// uploads only when an address, a size or the sampled values differ.
+static int resident_fofo(hipStream_t s, const double* jh, size_t jn, const double* kh, size_t kn,
+                         const double** dj, const double** dk)
+{
+    State& st = state();
+    double probe[2][NPROBE];
+    probe_values(jh, (int64_t)jn, probe[0]);
+    probe_values(kh, (int64_t)kn, probe[1]);
+    const bool same = st.fofo_key[0] == jh && st.fofo_key[1] == kh && st.fofo_n[0] == jn && st.fofo_n[1] == kn &&
+                      std::memcmp(probe, st.fofo_probe, sizeof(probe)) == 0;
+    if (!same) {
+        st.fofo_key[0] = st.fofo_key[1] = nullptr;
+        st.sys_valid = false;
+        GMC_TRY(st.fofo.reserve(carve_bytes({jn * 8, kn * 8})));
+        Arena ar{static_cast<char*>(st.fofo.ptr), st.fofo.bytes};
+        double* J = ar.take<double>(jn);
+        double* K = ar.take<double>(kn);
+        GMC_CHECK(ar.fits(), "ACEInteg_FOFO: layout overflow");
+        GMC_HIP(hipMemcpyAsync(J, jh, jn * 8, hipMemcpyHostToDevice, s));
+        GMC_HIP(hipMemcpyAsync(K, kh, kn * 8, hipMemcpyHostToDevice, s));
+        GMC_HIP(hipStreamSynchronize(s));
+        st.fofo_key[0] = jh;
+        st.fofo_key[1] = kh;
+        st.fofo_n[0] = jn;
+        st.fofo_n[1] = kn;
+        std::memcpy(st.fofo_probe, probe, sizeof(probe));
+    }
+    Arena ar{static_cast<char*>(st.fofo.ptr), st.fofo.bytes};
+    *dj = ar.take<double>(jn);
+    *dk = ar.take<double>(kn);
+    return 0;
+}
+
 // Stage the small arrays of one system in `ws`.
 static int stage_small(hipStream_t s, Workspace& ws, int N, int nA, int ndimx, const double* occ, const double* xone,
                        const double* ure, const double* rdm2act, const int* igem, const int* indn, Staged& d)
@@ -137,6 +172,50 @@ static int host_system(hipStream_t s, int N, int nI, int nA, int ndimx, const do
     AcRun& r = g_run;
     GMC_TRY(ac_prepare(s, N, nI, nA, ndimx, d_tw, g_staged.occ, g_staged.igem, g_staged.xone,
                        ure ? g_staged.ure : nullptr, g_staged.rdm2act, g_staged.indn, r));
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    GMC_HIP(hipStreamSynchronize(s));
+    st.sys_key = h;
+    st.sys_valid = true;
+    *out = &r;
+    return 0;
+}
+
+// Prepared FOFO system for host inputs, cached like host_system.
+static int host_system_fofo(hipStream_t s, int N, int nI, int nA, int ndimx, const double* jints,
+                            const double* kints, const double* occ, const int* igem, const double* xone,
+                            const double* ure, const double* rdm2act, const int* indn, AcRun** out)
+{
+    State& st = state();
+    GMC_CHECK(N >= 2 && ndimx >= 1 && nI >= 0 && nA >= 0 && nI + nA >= 1 && nI + nA <= N,
+              "ACEInteg_FOFO: bad sizes N=%d nI=%d nA=%d", N, nI, nA);
+    const size_t nO = (size_t)(nI + nA);
+    const double *d_j = nullptr, *d_k = nullptr;
+    GMC_TRY(resident_fofo(s, jints, nO * nO * N * N, kints, (size_t)N * nO * N * nO, &d_j, &d_k));
+    const size_t na2 = (size_t)nA * nA;
+    uint64_t h = 1469598103934665603ull;
+    const int dims[5] = {N, nI, nA, ndimx, 0x464f464f};   // tagged apart from the in-core systems
+    h = fnv(h, dims, sizeof(dims));
+    h = fnv(h, &jints, sizeof(jints));
+    h = fnv(h, &kints, sizeof(kints));
+    h = fnv(h, occ, (size_t)N * 8);
+    h = fnv(h, igem, (size_t)N * 4);
+    h = fnv(h, xone, (size_t)N * (N + 1) / 2 * 8);
+    h = ure ? fnv(h, ure, (size_t)N * N * 8) : fnv(h, "I", 1);
+    if (nA > 0) h = fnv(h, rdm2act, na2 * (na2 + 1) / 2 * 8);
+    h = fnv(h, indn, (size_t)2 * ndimx * 4);
+    if (st.sys_valid && st.sys_key == h && st.fofo_key[0] == jints && st.fofo_key[1] == kints) {
+        *out = &g_run;
+        return 0;
+    }
+    st.sys_valid = false;
+    GMC_TRY(stage_small(s, st.sysio, N, nA, ndimx, occ, xone, ure, nA > 0 ? rdm2act : nullptr, igem, indn,
+                        g_staged));
+    AcRun& r = g_run;
+    GMC_TRY(ac_prepare(s, N, nI, nA, ndimx, nullptr, g_staged.occ, g_staged.igem, g_staged.xone,
+                       ure ? g_staged.ure : nullptr, g_staged.rdm2act, g_staged.indn, r));
+    r.sys.J = d_j;
+    r.sys.K = d_k;
     GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
     GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
     GMC_HIP(hipStreamSynchronize(s));
@@ -319,16 +398,13 @@ int gmc_aceneerpa(double* ecorr, const double* eigvecr, const double* eig, const
     return 0;
 }
 
-int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, int64_t ninte2, const double* ure,
-                 const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
-                 int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
-                 double* eigvecr, double* eig)
+}  // extern "C"
+
+// One alpha point of a prepared system: W(alpha), ECASSCF and, in FP64 mode,
+// ERPASYMM1's side outputs (ACEInteg, acfd.f:4832-4966).
+static int aceinteg_point(hipStream_t s, AcRun* r, int prec, int ndimx, double acalpha, double* ecorr,
+                          double* egone, double* abplus, double* abmin, double* eigvecr, double* eig)
 {
-    GMC_TRY(ensure_init());
-    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACEInteg: unknown precision mode %d", prec);
-    hipStream_t s = state().stream;
-    AcRun* r = nullptr;
-    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
     double W = 0.0;
     const bool side = abplus || abmin || eigvecr || eig;
     if (prec == GMC_FP64 && side) {
@@ -364,6 +440,35 @@ int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, in
     *ecorr = W;
     if (egone) *egone = r->ab.ecasscf;
     return 0;
+}
+
+extern "C" {
+
+int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, int64_t ninte2, const double* ure,
+                 const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
+                 int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
+                 double* eigvecr, double* eig)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACEInteg: unknown precision mode %d", prec);
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
+    return aceinteg_point(s, r, prec, ndimx, acalpha, ecorr, egone, abplus, abmin, eigvecr, eig);
+}
+
+int gmc_aceinteg_fofo(int prec, double* ecorr, double* egone, const double* jints, const double* kints,
+                      const double* ure, const double* occ, const double* xone, const int* igem,
+                      const double* rdm2act, const int* indn, int ndimx, int nbasis, int ninact, int nact,
+                      double acalpha, double* abplus, double* abmin, double* eigvecr, double* eig)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACEInteg_FOFO: unknown precision mode %d", prec);
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system_fofo(s, nbasis, ninact, nact, ndimx, jints, kints, occ, igem, xone, ure, rdm2act, indn,
+                             &r));
+    return aceinteg_point(s, r, prec, ndimx, acalpha, ecorr, egone, abplus, abmin, eigvecr, eig);
 }
 
 int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono, int64_t ninte2,

@@ -25,6 +25,9 @@ struct State {
     const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
     int64_t resident_n = 0;
     double resident_probe[64] = {0};
+    const void* fofo_key[2] = {nullptr, nullptr};   // host (FF|OO), (FO|FO) arrays mirrored in `fofo`
+    size_t fofo_n[2] = {0, 0};
+    double fofo_probe[2][64] = {{0}};
     bool sys_valid = false;               // acws/sysio hold the system hashed as sys_key
     uint64_t sys_key = 0;
     void* blas = nullptr;                 // rocBLAS handle (blas.hip)
@@ -48,6 +51,7 @@ struct State {
         fofo = Workspace{};
         resident_key = nullptr;
         resident_n = 0;
+        fofo_key[0] = fofo_key[1] = nullptr;
         sys_valid = false;
     }
 };

@@ -1,7 +1,8 @@
 C     Drop-in replacements of the reference's AC-CAS routines, with the
 C     reference's own names and argument lists: TwoNO1 (misc.f:166),
 C     AB_CAS (interpa.f:3659), ERPASYMM1 (interpa.f:1652), ACEneERPA
-C     (acfd.f:4969) and ACEInteg (acfd.f:4832).  Linked into the reference
+C     (acfd.f:4969), ACEInteg (acfd.f:4832) and ACEInteg_FOFO
+C     (abfofo.f90:3664).  Linked into the reference
 C     program ahead of (or instead of) its own definitions, they read the
 C     same COMMON blocks (commons.inc of the reference tree) and the same
 C     rdm2.dat, and hand the work to libgammcor_hip through the C ABI.
@@ -136,5 +137,77 @@ C     CICoef as AB_CAS leaves it in /APSG/: sqrt(n), negative for n < 1/2
       Do I=1,NBasis
       CICoef(I)=Sign(Sqrt(Occ(I)),Occ(I)-0.5D0)
       EndDo
+      Return
+      End
+C
+C     A whole file of fixed-length records (the reference's direct-access
+C     FFOO and FOFO files carry no record markers) into X(1:N).
+      Subroutine GmcReadRecords(FName,X,N)
+      Implicit Real*8 (A-H,O-Z)
+      Character(*) FName
+      Integer(8) N,ISize
+      Dimension X(N)
+      Inquire(File=FName,Size=ISize)
+      If(ISize.Ne.8*N) Then
+      Write(6,'(3A,I0,A,I0)') ' gammcor_hip: ',FName,' has ',ISize,
+     $ ' bytes, expected ',8*N
+      Stop 1
+      EndIf
+      Open(NewUnit=IU,File=FName,Access='STREAM',Form='UNFORMATTED',
+     $ Status='OLD')
+      Read(IU) X
+      Close(IU)
+      Return
+      End
+C
+C     The CASSCF branch of ACEInteg_FOFO with NoSt = 1 (abfofo.f90:
+C     3712-3719, 3767-3786).  The two integral classes are read from the
+C     FFOO and FOFO record files that tran4_gen wrote (tran.f90:313-423):
+C     record k+(l-1)*NOcc of FFOO holds (kl|ij) for all i,j, record
+C     k+(l-1)*NBasis of FOFO holds (kl|ij) for all i and occupied j.  The
+C     reference defines this routine inside MODULE abfofo, so a caller is
+C     bound to this external one at link time (the module's symbol renamed
+C     in the caller's object, or the module's copy removed).
+      Subroutine ACEInteg_FOFO(ECorr,URe,Occ,XOne,UNOAO,
+     $ ABPLUS,ABMIN,EigVecR,Eig,EGOne,NGOcc,CICoef,
+     $ NBasis,NInte1,NDim,NGem,IndAux,ACAlpha,
+     $ IGemIN,NAct,INActive,NELE,IndN,IndX,NDimX,
+     $ NoSt,ICASSCF,IFlFrag1,IFunSR,IFunSRKer)
+      use gammcor_hip
+      Implicit Real*8 (A-H,O-Z)
+      Dimension URe(NBasis,NBasis),Occ(NBasis),XOne(NInte1),
+     $ UNOAO(NBasis,NBasis),ABPLUS(NDim*NDim),ABMIN(NDim*NDim),
+     $ EigVecR(NDimX*NDimX),Eig(NDimX),EGOne(NGem),CICoef(NBasis),
+     $ IndAux(NBasis),IGemIN(NBasis),IndN(2,NDim),IndX(NDim)
+      Real*8, Allocatable :: RDM2Act(:)
+      Real*8, Allocatable, Save :: FFOO(:),FOFO(:)
+      Integer(8) NJ,NK
+      Logical, Save :: Said = .False.
+      If(ICASSCF.Ne.1.Or.IFlFrag1.Ne.0.Or.NoSt.Ne.1.Or.
+     $ (IFunSR.Ne.0.And.IFunSRKer.Eq.1)) Then
+      Write(6,'(1X,A)') 'gammcor_hip: ACEInteg_FOFO supports '//
+     $ 'ICASSCF=1, IFlFrag1=0, NoSt=1 and no SR kernel'
+      Stop 1
+      EndIf
+      Call GmcCheckIndX(IndX,NDimX,NDim,'ACEInteg_FOFO')
+      NOcc=INActive+NAct
+      NJ=Int(NOcc,8)**2*Int(NBasis,8)**2
+      NK=NJ
+      If(Allocated(FFOO)) Then
+      If(Size(FFOO,Kind=8).Ne.NJ) Deallocate(FFOO,FOFO)
+      EndIf
+      If(.Not.Allocated(FFOO)) Allocate(FFOO(NJ),FOFO(NK))
+      Call GmcReadRecords('FFOO',FFOO,NJ)
+      Call GmcReadRecords('FOFO',FOFO,NK)
+      IPrec=gmc_prec_from_env()
+      If(.Not.Said) Write(6,'(1X,A,I0)')
+     $ 'gammcor_hip: ACEInteg_FOFO on the GPU, precision mode ',IPrec
+      Said=.True.
+      Allocate(RDM2Act(Max(1,NAct**2*(NAct**2+1)/2)))
+      Call GmcRdm2Act(RDM2Act,NAct)
+      Call gmc_check(gmc_aceinteg_fofo(IPrec,ECorr,EGOne(1),FFOO,FOFO,
+     $ URe,Occ,XOne,IGemIN,RDM2Act,IndN,NDimX,NBasis,INActive,NAct,
+     $ ACAlpha,ABPLUS,ABMIN,EigVecR,Eig),'ACEInteg_FOFO')
+      Deallocate(RDM2Act)
       Return
       End

@@ -243,6 +243,31 @@ def ACEInteg(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, a
     return res
 
 
+def ACEInteg_FOFO(jints, kints, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, acalpha,
+                  prec=GMC_FP64, side_outputs=False):
+    """One alpha point of the FOFO variant (abfofo.f90:3664): jints and kints
+    are the (FF|OO) and (FO|FO) classes in the record order of the FFOO and
+    FOFO files.  Returns the same dict as ACEInteg."""
+    indn, ig = _i(indn), _i(igem)
+    oc, xo, rd = _f(occ), _f(xone), _f(rdm2act)
+    jj, kk = _f(jints), _f(kints)
+    u = None if ure is None else _f(ure)
+    nd = len(indn)
+    nO = ninact + nact
+    if jj.size != nO * nO * nbasis * nbasis or kk.size != nbasis * nO * nbasis * nO:
+        raise ValueError("ACEInteg_FOFO: jints/kints sizes do not match NBasis and the occupied count")
+    ec = np.zeros(1)
+    eg = np.zeros(1)
+    outs = [np.zeros((nd, nd), order="F") for _ in range(3)] + [np.zeros(nd)] if side_outputs else [None] * 4
+    _lib.call("gmc_aceinteg_fofo", prec, ec.ctypes.data, eg.ctypes.data, jj.ctypes.data, kk.ctypes.data, _ptr(u),
+              oc.ctypes.data, xo.ctypes.data, ig.ctypes.data, rd.ctypes.data, indn.ctypes.data, nd, nbasis, ninact,
+              nact, float(acalpha), *[_ptr(o) for o in outs])
+    res = dict(ecorr=float(ec[0]), egone=float(eg[0]))
+    if side_outputs:
+        res.update(abplus=outs[0], abmin=outs[1], eigvecr=outs[2], eig=outs[3])
+    return res
+
+
 def ACECORR_host(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2act, alphas, weights,
                  prec=GMC_FP64, alpha_batch=0):
     """ACECORR's alpha loop with host inputs (acfd.f:161-201)."""

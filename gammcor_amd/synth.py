@@ -296,6 +296,21 @@ def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=
     return sysm
 
 
+def fofo_records(twono, nbasis, nocc):
+    """The (FF|OO) and (FO|FO) integral classes in the record order of the
+    reference's FFOO and FOFO files (tran4_gen, tran.f90:313-423), taken from
+    packed NO integrals: J[(k + nO l) N^2 + i + N j] = (kl|ij) with k, l
+    occupied; K[(f + N o) N nO + f' + N o'] = (fo|f'o') with o, o' occupied."""
+    n, no = nbasis, nocc
+    i = np.arange(n)
+    o = np.arange(no)
+    l_, k_, j_, i_ = np.meshgrid(o, o, i, i, indexing="ij")
+    jrec = np.asarray(twono)[naddr3(k_, l_, i_, j_)]
+    o_, f_, o2, f2 = np.meshgrid(o, i, o, i, indexing="ij")
+    krec = np.asarray(twono)[naddr3(f_, o_, f2, o2)]
+    return np.ascontiguousarray(jrec.reshape(-1)), np.ascontiguousarray(krec.reshape(-1))
+
+
 def config_system(name, with_integrals=True):
     cfg = dict(CONFIGS[name])
     return make_system(with_integrals=with_integrals, **cfg)

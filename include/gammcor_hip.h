@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define GMC_ABI_VERSION 2
+#define GMC_ABI_VERSION 3
 /* arithmetic modes of the AC-CAS solve */
 #define GMC_FP64 0    /* plain double precision throughout                     */
 #define GMC_FP32 1    /* eigen solves in float, no refinement (diagnostic)     */
@@ -84,6 +84,23 @@ int gmc_aceinteg(int prec, double* ecorr, double* egone, const double* twono, in
                  const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
                  int ndimx, int nbasis, int ninact, int nact, double acalpha, double* abplus, double* abmin,
                  double* eigvecr, double* eig);
+
+/* ACEInteg_FOFO(ECorr,URe,Occ,XOne,UNOAO,ABPLUS,ABMIN,EigVecR,Eig,EGOne,
+ * NGOcc,CICoef,NBasis,NInte1,NDim,NGem,IndAux,ACAlpha,IGemIN,NAct,INActive,
+ * NELE,IndN,IndX,NDimX,NoSt,ICASSCF,IFlFrag1,IFunSR,IFunSRKer) -
+ * abfofo.f90:3664-3789, its CASSCF branch with NoSt = 1: AB_CAS_FOFO ->
+ * ERPASYMM1 -> ACEneERPA_FOFO at one alpha.  The integrals are the two
+ * classes the reference reads from its FFOO and FOFO record files
+ * (tran4_gen, tran.f90:313-423), in record order, nO = ninact + nact:
+ *   jints[(k + nO l) N^2 + i + N j]     = (kl|ij),   k, l < nO   (FFOO)
+ *   kints[(f + N o) N nO + f' + N o']   = (fo|f'o'), o, o' < nO (FOFO)
+ * Both stay on the device across calls (keyed by address, size and sampled
+ * values), so the later alpha points of ACECORR cost only the ERPA solve.
+ * Outputs as gmc_aceinteg. */
+int gmc_aceinteg_fofo(int prec, double* ecorr, double* egone, const double* jints, const double* kints,
+                      const double* ure, const double* occ, const double* xone, const int* igem,
+                      const double* rdm2act, const int* indn, int ndimx, int nbasis, int ninact, int nact,
+                      double acalpha, double* abplus, double* abmin, double* eigvecr, double* eig);
 
 /* ACECORR's alpha loop (acfd.f:161-201) with host inputs: all alpha points
  * batched (alpha_batch points per solve, 0 = 16), ecorr = sum w_i W(alpha_i),

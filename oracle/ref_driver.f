@@ -8,11 +8,17 @@ C               eigenvalue summaries at full precision;
 C       accas : the reference's ACCAS (accas.f:2) end to end, with its own
 C               pair selection and 5-point GauLeg (acfd.f:120);
 C       dumpab: AB_CAS (interpa.f:3659) at the first alpha node, writing
-C               ABPLUS/ABMIN to abplus.bin/abmin.bin.
+C               ABPLUS/ABMIN to abplus.bin/abmin.bin;
+C       fofo  : the FOFO variant: the (FF|OO) and (FO|FO) classes of the
+C               transformed integrals written as the FFOO and FOFO record
+C               files tran4_gen produces (tran.f90:313-423), then
+C               ACEInteg_FOFO (abfofo.f90:3664) at each alpha node, printing
+C               as grid does.
 C     Pair selection for grid/dumpab mirrors accas.f:30-99 with
 C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
       Program RefDrv
       use types
+      use abfofo, only : ACEInteg_FOFO
       Implicit Real*8 (A-H,O-Z)
       Include 'commons.inc'
       Character*60 Title
@@ -168,13 +174,43 @@ C
       Stop
       EndIf
 C
+      If(Mode(1:4).Eq.'fofo') Then
+      NOcc=NInAc+NAct
+      Open(14,File='FFOO',Status='REPLACE',Access='DIRECT',
+     $ Form='UNFORMATTED',Recl=8*NBasis*NBasis)
+      Do L=1,NOcc
+      Do K=1,NOcc
+      Write(14,Rec=K+(L-1)*NOcc) ((TwoNO(NAddr3(K,L,I,J)),I=1,NBasis),
+     $ J=1,NBasis)
+      EndDo
+      EndDo
+      Close(14)
+      Open(14,File='FOFO',Status='REPLACE',Access='DIRECT',
+     $ Form='UNFORMATTED',Recl=8*NBasis*NOcc)
+      Do L=1,NOcc
+      Do K=1,NBasis
+      Write(14,Rec=K+(L-1)*NBasis) ((TwoNO(NAddr3(K,L,I,J)),I=1,NBasis),
+     $ J=1,NOcc)
+      EndDo
+      EndDo
+      Close(14)
+      EndIf
+C
       ECorr=0.D0
       Do IA=1,NAl
+      If(Mode(1:4).Eq.'fofo') Then
+      Call ACEInteg_FOFO(ECorrA,URe,Occ,XOne,UNOAO,
+     $ ABPLUS,ABMIN,EigVecR,Eig,EGOne,NGOcc,CICoef,
+     $ NBasis,NInte1,NDimX,NGem,IndAux,AG(IA),
+     $ IGem,NAcCAS,NInAcCAS,NELE,IndN,IndX,NDimX,
+     $ NoSt,ICASSCF,IFlFrag1,IFunSR,IFunSRKer)
+      Else
       Call ACEInteg(ECorrA,TwoNO,URe,Occ,XOne,UNOAO,
      $ ABPLUS,ABMIN,EigVecR,Eig,
      $ EGOne,NGOcc,
      $ Title,NBasis,NInte1,NInte2,NDimX,NGem,IndAux,AG(IA),
      $ IndN,IndX,NDimX)
+      EndIf
       ECorr=ECorr+WG(IA)*ECorrA
       ES=0.D0
       NPos=0

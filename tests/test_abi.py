@@ -31,4 +31,4 @@ def test_library_exports_every_declared_symbol():
 def test_version_without_gpu():
     from gammcor_amd import _lib
     lib = ctypes.CDLL(_lib.LIB_PATH)
-    assert lib.gmc_version() == 2
+    assert lib.gmc_version() == 3

@@ -115,6 +115,31 @@ def test_aceinteg_host_matches_reference(cuda, name):
             assert abs(w - w_ref) < 1e-9
 
 
+@pytest.mark.parametrize("name", ["n16_cas22", "n20_cas44_noinact", "n24_cas66"])
+def test_aceinteg_fofo_host_matches_reference(cuda, name):
+    """ACEInteg_FOFO from the (FF|OO) and (FO|FO) record arrays: every W(alpha)
+    and ECASSCF of the reference fixture (the reference's own FOFO path gives
+    the in-core values to 1e-13 on these systems), and in FP64 mode the same
+    ERPASYMM1 side outputs as the in-core drop-in."""
+    from gammcor_amd import host, synth
+    fx = gold(name)
+    s = _system(fx["system"])
+    tno, indn, igem, rdm = _inputs(s)
+    jrec, krec = synth.fofo_records(tno, s.nbasis, s.ninact + s.nact)
+    print(f"drop-in ACEInteg_FOFO {name} ndimx={len(indn)}", flush=True)
+    for i, (a, w_ref) in enumerate(zip(fx["alphas"], fx["walpha"])):
+        r = host.ACEInteg_FOFO(jrec, krec, None, s.occ, s.xone, indn, s.nbasis, s.ninact, s.nact, igem, rdm, a,
+                               side_outputs=(i == 0))
+        assert abs(r["ecorr"] - w_ref) < 1e-9
+        assert abs(r["egone"] - fx["ecasscf"]) < 1e-10
+        if i == 0:
+            q = host.ACEInteg(tno, None, s.occ, s.xone, indn, s.nbasis, s.ninact, s.nact, igem, rdm, a,
+                              side_outputs=True)
+            assert np.max(np.abs(r["eig"] - q["eig"])) < 1e-10 * max(1.0, np.max(np.abs(q["eig"])))
+            assert np.max(np.abs(r["abplus"] - q["abplus"])) < 1e-10 * max(1.0, np.max(np.abs(q["abplus"])))
+            assert np.max(np.abs(r["abmin"] - q["abmin"])) < 1e-10 * max(1.0, np.max(np.abs(q["abmin"])))
+
+
 def test_acecorr_host_matches_reference_and_tracks_new_integrals(cuda):
     from gammcor_amd import host
     fx = gold("n48_cas44")

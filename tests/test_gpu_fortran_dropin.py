@@ -1,9 +1,9 @@
 """The reference program with the Fortran drop-in layer linked in place of its
-own TwoNO1, AB_CAS, ERPASYMM1, ACEneERPA and ACEInteg (oracle/_ref/
-ref_driver_gpu, built by `make -C oracle ref_gpu` from the reference sources
-plus gammcor_amd/fortran): the reference's own ACECORR and driver run
-unchanged and call the GPU through iso_c_binding.  Its printed results must
-match the CPU reference's fixtures on the same seeded inputs."""
+own TwoNO1, AB_CAS, ERPASYMM1, ACEneERPA, ACEInteg and ACEInteg_FOFO
+(oracle/_ref/ref_driver_gpu, built by `make -C oracle ref_gpu` from the
+reference sources plus gammcor_amd/fortran): the reference's own ACECORR and
+driver run unchanged and call the GPU through iso_c_binding.  Its printed
+results must match the CPU reference's fixtures on the same seeded inputs."""
 import os
 import subprocess
 import sys
@@ -65,3 +65,17 @@ def test_reference_grid_driver_mixed_precision(cuda):
     fx, res, out = _run("n24_cas66", "grid", prec="mixed")
     assert "precision mode 2" in out
     assert abs(res["ecorr"] - fx["ecorr"]) < 1e-6
+
+
+@needs_driver
+@pytest.mark.parametrize("name", ["n16_cas22", "n20_cas44_noinact"])
+def test_reference_fofo_driver_through_the_fortran_drop_in(cuda, name):
+    """The driver writes the FFOO and FOFO record files from the transformed
+    integrals and calls ACEInteg_FOFO at every node; the drop-in reads the
+    files and runs on the GPU.  The reference's own FOFO path reproduces the
+    in-core fixture values to 1e-13 on these systems."""
+    fx, res, out = _run(name, "fofo")
+    assert "gammcor_hip: ACEInteg_FOFO on the GPU" in out
+    assert np.max(np.abs(np.array(res["walpha"]) - np.array(fx["walpha"]))) < 1e-9
+    assert abs(res["ecorr"] - fx["ecorr"]) < 1e-9
+    assert abs(res["ecasscf"] - fx["ecasscf"]) < 1e-10
