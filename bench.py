@@ -134,8 +134,8 @@ def main():
                          "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, see gammcor_amd.profsum)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the path's "
+              "multi-GPU sharding (alpha points per GPU, one 8-byte all-reduce) is not run.", file=sys.stderr)
         sys.exit(2)
 
     import torch
