@@ -30,18 +30,7 @@ def build_inputs(name, device):
     from gammcor_amd import synth
     cfg = dict(synth.CONFIGS[name])
     s = synth.make_system(with_integrals=False, **cfg)
-    n = s.nbasis
-    t = n * (n + 1) // 2
-    chol = torch.from_numpy(s.chol).to(device)
-    ao = torch.empty(t * (t + 1) // 2, dtype=torch.float64, device=device)
-    blk = 1024
-    cols = torch.arange(t, device=device)
-    for r0 in range(0, t, blk):
-        r1 = min(t, r0 + blk)
-        block = chol[:, r0:r1].T @ chol[:, :r1]
-        rows = torch.arange(r0, r1, device=device)[:, None]
-        mask = cols[None, :r1] <= rows
-        ao[r0 * (r0 + 1) // 2: r1 * (r1 + 1) // 2] = block[mask]
+    ao = synth.ao_integrals_device(s.chol, device)
     torch.cuda.synchronize()
     return s, cfg, ao
 

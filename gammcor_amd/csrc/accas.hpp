@@ -70,10 +70,14 @@ size_t integrand_from_y_bytes(int n);
 int integrand_from_y(hipStream_t st, int n, const double* Y, const double* eig, const double* Gw, const double* wdiag,
                      void* scratch, double* W);
 
-// Mixed path: FP32 eigen solves refined by n_oa Ogita-Aishima steps in FP64.
+// Largest number of Ogita-Aishima steps of the adaptive mixed path.
+constexpr int OA_MAX_STEPS = 6;
+
+// Mixed path: FP32 eigen solves refined by n_min to n_max Ogita-Aishima
+// steps in FP64 (more than n_min while the corrections stay large).
 size_t erpa_mixed_bytes(int n, int batch);
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_oa);
+                       int n_min, int n_max);
 
 }  // namespace gmc

@@ -67,6 +67,12 @@ int State::eig_plan_for(int n, int batch, EigPlan<T>** out)
 template int State::eig_plan_for<double>(int, int, EigPlan<double>**);
 template int State::eig_plan_for<float>(int, int, EigPlan<float>**);
 
+// The stream of a device-pointer entry point: the caller's, and for NULL the
+// null stream itself, so the call is ordered after the caller's earlier
+// default-stream work (the library's own stream is non-blocking and would
+// not be).
+static hipStream_t dev_stream(void* stream) { return static_cast<hipStream_t>(stream); }
+
 int ensure_init()
 {
     State& st = state();
@@ -180,8 +186,11 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             if (prec == GMC_FP64) GMC_TRY(run(double(0)));
             else GMC_TRY(run(float(0)));
         } else {
-            int n_oa = prec == GMC_MIXED2 ? 2 : 3;
-            if (const char* e = getenv("GMC_OA_STEPS")) n_oa = std::max(0, atoi(e));
+            // GMC_MIXED: at least three steps, more (up to OA_MAX_STEPS) while
+            // the corrections stay large; GMC_MIXED2: exactly two;
+            // GMC_OA_STEPS fixes the count (diagnostics)
+            int n_min = prec == GMC_MIXED2 ? 2 : 3, n_max = prec == GMC_MIXED2 ? 2 : OA_MAX_STEPS;
+            if (const char* e = getenv("GMC_OA_STEPS")) n_min = n_max = std::max(0, atoi(e));
             EigPlan<float>* ep = nullptr;
             GMC_TRY(st.eig_plan_for<float>(nd, B, &ep));
             GMC_TRY(st.scratch.reserve(ep->ws_bytes));
@@ -189,7 +198,8 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             const size_t mb = erpa_mixed_bytes(nd, B);
             GMC_TRY(st.erpaws.reserve(align_up(mb) + outb));
             bind_out(align_up(mb));
-            GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_oa));
+            GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_min,
+                                           n_max));
             GMC_TRY(eig_check_error<float>(s, *ep));
         }
         GMC_HIP(hipMemcpyAsync(walpha + a0, out.W, B * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -247,7 +257,7 @@ int gmc_twono1_dev(double* tno, const double* ure, int nbasis, int64_t ninte2, v
               (long long)ninte2, nbasis);
     State& st = state();
     GMC_TRY(st.scratch.reserve(p.scratch_bytes));
-    hipStream_t s = stream ? (hipStream_t)stream : st.stream;
+    hipStream_t s = dev_stream(stream);
     GMC_TRY(twono1_device(s, p, tno, ure, st.scratch.ptr));
     if (!stream) GMC_HIP(hipStreamSynchronize(s));
     return 0;
@@ -259,7 +269,7 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
 {
     GMC_TRY(ensure_init());
     GemmShape g{m, n, k, ta != 0, tb != 0, lda, ldb, ldc, sa, sb, sc, batch, 0};
-    hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    hipStream_t s = dev_stream(stream);
     if (prec == GMC_FP64)
         GMC_TRY(gemm<double>(s, g, alpha, (const double*)A, (const double*)B, beta, (double*)C));
     else
@@ -271,7 +281,7 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream)
 {
     GMC_TRY(ensure_init());
-    hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    hipStream_t s = dev_stream(stream);
     if (prec == GMC_FP64) return syevd_dev_impl<double>(n, batch, (double*)A, (double*)w, s);
     return syevd_dev_impl<float>(n, batch, (float*)A, (float*)w, s);
 }
@@ -284,7 +294,7 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
     GMC_TRY(ensure_init());
     GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR: bad sizes");
     GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR: unknown precision mode %d", prec);
-    hipStream_t s = stream ? (hipStream_t)stream : state().stream;
+    hipStream_t s = dev_stream(stream);
     state().sys_valid = false;   // acws is overwritten: drop the host-call system cache
     AcRun r;
     GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
@@ -312,7 +322,7 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
     GMC_CHECK(ninact + nact >= 1, "ACECORR FOFO: no occupied orbitals");
     GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR FOFO: unknown precision mode %d", prec);
     State& st = state();
-    hipStream_t s = stream ? (hipStream_t)stream : st.stream;
+    hipStream_t s = dev_stream(stream);
     st.sys_valid = false;   // acws and fofo are overwritten: drop the host-call caches
     st.fofo_key[0] = st.fofo_key[1] = nullptr;
     FofoPlan fp;

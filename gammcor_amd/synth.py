@@ -311,6 +311,45 @@ def fofo_records(twono, nbasis, nocc):
     return np.ascontiguousarray(jrec.reshape(-1)), np.ascontiguousarray(krec.reshape(-1))
 
 
+# The batching workload: 64 independent systems of uneven size, NBasis uniform
+# in [40, 140] from seed 3, NBasis/12 inactive orbitals, CAS(4e,4o) and
+# CAS(6e,6o) alternating, rank 3 NBasis integrals.  The description leaves
+# sigma and the per-system seeds open: sigma 0.1 as for the NBasis=48 system,
+# system k seeded 1000 + k.
+BATCH = dict(count=64, nbasis_range=(40, 140), seed=3, sigma=0.1, nalpha=16, seed0=1000)
+
+
+def batch_configs(count=BATCH["count"], seed=BATCH["seed"]):
+    """Per-system keyword arguments of make_system for the batching workload."""
+    rng = np.random.default_rng(seed)
+    lo, hi = BATCH["nbasis_range"]
+    out = []
+    for k in range(count):
+        n = int(rng.integers(lo, hi + 1))
+        na = 4 if k % 2 == 0 else 6
+        out.append(dict(nbasis=n, ninact=n // 12, nact=na, nelact=na, rank=3 * n, sigma=BATCH["sigma"],
+                        seed=BATCH["seed0"] + k))
+    return out
+
+
+def ao_integrals_device(chol, device):
+    """The packed AO integrals (mn|ls) = sum_P L_Pmn L_Pls of ao_integrals_packed,
+    formed on the device in row blocks of the pair matrix (torch)."""
+    import torch
+    t = chol.shape[1]
+    c = torch.from_numpy(np.ascontiguousarray(chol)).to(device)
+    ao = torch.empty(t * (t + 1) // 2, dtype=torch.float64, device=device)
+    blk = 1024
+    cols = torch.arange(t, device=device)
+    for r0 in range(0, t, blk):
+        r1 = min(t, r0 + blk)
+        block = c[:, r0:r1].T @ c[:, :r1]
+        rows = torch.arange(r0, r1, device=device)[:, None]
+        mask = cols[None, :r1] <= rows
+        ao[r0 * (r0 + 1) // 2: r1 * (r1 + 1) // 2] = block[mask]
+    return ao
+
+
 def config_system(name, with_integrals=True):
     cfg = dict(CONFIGS[name])
     return make_system(with_integrals=with_integrals, **cfg)

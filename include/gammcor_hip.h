@@ -6,9 +6,10 @@
  *     returns the message of the calling thread's last failure;
  *   - matrices are column-major (Fortran order), index arrays are 1-based
  *     exactly as the reference passes them;
- *   - "_dev" entry points take device pointers and an optional HIP stream
- *     (NULL = the library stream, synchronised before return); the others
- *     take host pointers and are synchronous;
+ *   - "_dev" entry points take device pointers and a HIP stream to run on;
+ *     NULL is the null (default) stream, which orders the call after the
+ *     caller's earlier default-stream work, and is synchronised before
+ *     return; the others take host pointers and are synchronous;
  *   - the library is not re-entrant (neither are the reference routines).
  */
 #ifndef GAMMCOR_HIP_H
@@ -23,7 +24,8 @@ extern "C" {
 /* arithmetic modes of the AC-CAS solve */
 #define GMC_FP64 0    /* plain double precision throughout                     */
 #define GMC_FP32 1    /* eigen solves in float, no refinement (diagnostic)     */
-#define GMC_MIXED 2   /* float eigen solves + three Ogita-Aishima steps in double */
+#define GMC_MIXED 2   /* float eigen solves + Ogita-Aishima steps in double: three, */
+                      /* up to six while the corrections stay large             */
 #define GMC_MIXED2 3  /* float eigen solves + two Ogita-Aishima steps (faster,    */
                       /* misses 1 uEh at NBasis=240; diagnostic)                 */
 

@@ -81,3 +81,30 @@ def test_n160_fp64_matches_reference(cuda):
     assert np.max(np.abs(r["walpha"] - np.array(fx["walpha"]))) < 1e-9
     m = host.ACECORR(d, fx["alphas"], fx["weights"], prec=host.GMC_MIXED)
     assert abs(m["ecorr"] - fx["ecorr"]) < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLD, "ref_n160_cas88.json")), reason="fixture not generated")
+def test_device_entries_follow_the_callers_default_stream(cuda):
+    """The device-pointer entry points with stream NULL run on the null stream:
+    the caller's default-stream copy of the AO integrals into the work array
+    is complete before the in-place transform reads it.  Regression test for
+    a NULL stream that had meant the library's non-blocking stream, where a
+    mixed solve followed by an FP64 solve with eigenvalue output came out
+    wrong at NBasis=160 (the bench's sequence, with its in-place transform)."""
+    import torch
+    from gammcor_amd import host, synth, _lib
+    fx = gold("n160_cas88")
+    m = fx["system"]
+    s = synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"],
+                          with_integrals=False)
+    ao = synth.ao_integrals_device(s.chol, cuda)
+    work = torch.empty_like(ao)
+    ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(cuda)
+    d = host.DeviceSystem(work, s.occ, s.xone, s.gamma_act, s.nbasis, s.ninact, s.nact, s.nele, device=cuda)
+    print("default-stream ordering n160", flush=True)
+    for prec, tol in ((host.GMC_MIXED, 1e-6), (host.GMC_FP64, 1e-9), (host.GMC_MIXED, 1e-6), (host.GMC_FP64, 1e-9)):
+        work.fill_(float("nan"))
+        work.copy_(ao)
+        _lib.call("gmc_twono1_dev", work.data_ptr(), ure.data_ptr(), s.nbasis, work.numel(), None)
+        r = host.ACECORR(d, fx["alphas"], fx["weights"], prec=prec, return_eig=True)
+        assert abs(r["ecorr"] - fx["ecorr"]) < tol, (prec, r["ecorr"] - fx["ecorr"])
