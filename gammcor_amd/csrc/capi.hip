@@ -259,7 +259,7 @@ int gmc_twono1_dev(double* tno, const double* ure, int nbasis, int64_t ninte2, v
     GMC_TRY(st.scratch.reserve(p.scratch_bytes));
     hipStream_t s = dev_stream(stream);
     GMC_TRY(twono1_device(s, p, tno, ure, st.scratch.ptr));
-    if (!stream) GMC_HIP(hipStreamSynchronize(s));
+    GMC_HIP(hipStreamSynchronize(s));   // the workspaces are reused by the next call
     return 0;
 }
 
@@ -274,7 +274,7 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
         GMC_TRY(gemm<double>(s, g, alpha, (const double*)A, (const double*)B, beta, (double*)C));
     else
         GMC_TRY(gemm<float>(s, g, (float)alpha, (const float*)A, (const float*)B, (float)beta, (float*)C));
-    if (!stream) GMC_HIP(hipStreamSynchronize(s));
+    GMC_HIP(hipStreamSynchronize(s));   // the workspaces are reused by the next call
     return 0;
 }
 

@@ -8,8 +8,9 @@
  *     exactly as the reference passes them;
  *   - "_dev" entry points take device pointers and a HIP stream to run on;
  *     NULL is the null (default) stream, which orders the call after the
- *     caller's earlier default-stream work, and is synchronised before
- *     return; the others take host pointers and are synchronous;
+ *     caller's earlier default-stream work.  They synchronise that stream
+ *     before returning (the library's workspaces serve the next call); the
+ *     others take host pointers and are synchronous;
  *   - the library is not re-entrant (neither are the reference routines).
  */
 #ifndef GAMMCOR_HIP_H
