@@ -747,9 +747,9 @@ constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below t
 // step's largest correction is at least OA_EXTEND, up to OA_MAX_STEPS
 // (accas.hpp): the
 // error left after a step scales with the square of its largest correction.
-// Measured over 64 systems of NBasis 40-140 and at NBasis 160 and 240, every
-// system whose third correction reached 9e-4 or more missed 1 uEh after three
-// steps, and every one at 2.6e-4 or less met it.
+// On 64 systems of NBasis 40-140 three steps missed 1 uEh on three, whose
+// third corrections were 9e-4 to 1.4e-2 (M); the NBasis=160 and 240 systems
+// had 9e-5 and 2.8e-4 and met it with three steps.
 constexpr float OA_EXTEND = 5e-4f;
 
 __device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
