@@ -306,3 +306,43 @@ def test_rdm2_dat_round_trip():
                     assert packed[mx * (mx - 1) // 2 + mn - 1] == back[i, j, k, l]
     # Gamma = D/2 with sum_pq D_pqpq = ne(ne-1): the trace is ne(ne-1)/2
     assert abs(np.einsum("pqpq->", gam) - 4 * 3 / 2) < 1e-12
+
+
+REF_DRIVER = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "ref_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="oracle/_ref/ref_driver not built (make -C oracle ref)")
+@pytest.mark.parametrize("name", ["n16_cas22", "n20_cas44_noinact"])
+def test_reference_fofo_path_on_the_record_layout(name):
+    """The compiled reference's own ACEInteg_FOFO (abfofo.f90:3664), fed the
+    FFOO and FOFO record files in the layout the GPU entry point takes
+    (written by the reference driver's fofo mode from the transformed
+    integrals), reproduces the in-core fixture: the record order of both
+    classes is the reference's."""
+    import subprocess
+    import sys
+    import tempfile
+    sys.path.insert(0, GOLD)
+    import make_golden as G
+    from gammcor_amd import synth
+    fx = json.load(open(os.path.join(GOLD, f"ref_{name}.json")))
+    m = fx["system"]
+    s = synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"])
+    alphas, weights = synth.gauss_legendre_01(fx["nalpha"])
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_inputs(s, d, alphas, weights)
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; {REF_DRIVER} fofo"], cwd=d, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        # the FFOO records hold (kl|ij) in the order of synth.fofo_records
+        from accas_oracle import two_no1
+        tw = two_no1(s.twoel_ao, s.uaomo)
+        jrec, krec = synth.fofo_records(tw, s.nbasis, s.ninact + s.nact)
+        jfile = np.fromfile(os.path.join(d, "FFOO"), dtype=np.float64)
+        kfile = np.fromfile(os.path.join(d, "FOFO"), dtype=np.float64)
+    res = G.parse(r.stdout)
+    assert np.max(np.abs(np.array(res["walpha"]) - np.array(fx["walpha"]))) < 1e-11
+    assert abs(res["ecorr"] - fx["ecorr"]) < 1e-11
+    assert jfile.shape == jrec.shape and kfile.shape == krec.shape
+    assert np.max(np.abs(jfile - jrec)) < 1e-12 * max(1.0, np.max(np.abs(jrec)))
+    assert np.max(np.abs(kfile - krec)) < 1e-12 * max(1.0, np.max(np.abs(krec)))
