@@ -156,6 +156,16 @@ struct alignas(16) Vec16 {
 };
 template <class T>
 __device__ inline Vec16<T> ld16(const T* p) { return *reinterpret_cast<const Vec16<T>*>(p); }
+// 16-byte buffer load (buffer_load_dwordx4 ... offen): the descriptor lives in
+// SGPRs, the row offset in one VGPR and the column offset in an SGPR, so the
+// columns of a group share one address register.  Word 3 0x00020000 is the
+// gfx9 raw-buffer format (32-bit data, untyped).
+constexpr int BUF_WORD3 = 0x00020000;
+template <class T>
+__device__ inline Vec16<T> bld16(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(Vec16<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
 
 template <class T, bool VLDS, bool PLDS, int KV, int UV>
 __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
@@ -333,13 +343,23 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                             if (c + e > j && c + e > r0) lds_add(&u[c + e], w[e]);
                     }
                 }
+                // rows below the diagonal block through one buffer descriptor
+                // over the group's KV columns (KV n elements from column r0);
+                // a column past n reads column r0 instead, as above
+                const int r0s = __builtin_amdgcn_readfirstlane(r0);
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    A + (size_t)r0s * n, 0, KV * n * (int)sizeof(T), BUF_WORD3);
+                int soff[KV];
+#pragma unroll
+                for (int kk = 0; kk < KV; kk++) soff[kk] = (r0s + kk < n ? kk : 0) * n * (int)sizeof(T);
                 int c = base + 64 * VN + VN * lane;
                 for (; c + (UV - 1) * 64 * VN < n; c += UV * 64 * VN) {
                     Vec16<T> a[UV][KV], vv[UV];
 #pragma unroll
                     for (int uu = 0; uu < UV; uu++) {
 #pragma unroll
-                        for (int kk = 0; kk < KV; kk++) a[uu][kk] = ld16(col[kk] + c + uu * 64 * VN);
+                        for (int kk = 0; kk < KV; kk++)
+                            a[uu][kk] = bld16<T>(rs, (c + uu * 64 * VN) * (int)sizeof(T), soff[kk]);
                         vv[uu] = ld16(v + c + uu * 64 * VN);
                     }
 #pragma unroll
@@ -365,7 +385,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                     for (int e = 0; e < VN; e++) w[e] = 0;
 #pragma unroll
                     for (int kk = 0; kk < KV; kk++) {
-                        const Vec16<T> a = ld16(col[kk] + c);
+                        const Vec16<T> a = bld16<T>(rs, c * (int)sizeof(T), soff[kk]);
 #pragma unroll
                         for (int e = 0; e < VN; e++) {
                             s[kk] += a.x[e] * vv.x[e];
