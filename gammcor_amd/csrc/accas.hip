@@ -743,14 +743,6 @@ __global__ void oa_delta_kernel(const double* partial, int np, double* delta, in
 constexpr int OA_MAXPAIRS = 4096;
 constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
 constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
-// After the minimum number of steps the refinement goes on while the last
-// step's largest correction is at least OA_EXTEND, up to OA_MAX_STEPS
-// (accas.hpp): the
-// error left after a step scales with the square of its largest correction.
-// On 64 systems of NBasis 40-140 three steps missed 1 uEh on three, whose
-// third corrections were 9e-4 to 1.4e-2 (M); the NBasis=160 and 240 systems
-// had 9e-5 and 2.8e-4 and met it with three steps.
-constexpr float OA_EXTEND = 5e-4f;
 
 __device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
                           unsigned* npair, int4* pairs, double2* rot, int b, int i, int j);
@@ -915,15 +907,16 @@ size_t erpa_mixed_bytes(int n, int batch)
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
                         B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4,
-                        B * (size_t)OA_EMAX_SLOTS * 4});
+                        B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8});
 }
 
 // Mixed ERPA: eigen solves of A+ and of M = S A- S in FP32, each refined in
-// FP64 by n_min to n_max Ogita-Aishima iterations (n_min >= 1); S, M, Y and
+// FP64 by n_min to n_max Ogita-Aishima iterations (n_min >= 1), problems that
+// do not converge solved again in FP64 when `resolve` is given; S, M, Y and
 // the integrand in FP64.
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max)
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve)
 {
     GMC_CHECK(ep.bound && ep.batch == nalpha && ep.n == n, "ERPA mixed: eigensolver plan mismatch");
     const int B = nalpha;
@@ -951,6 +944,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     float* X32 = ar.take<float>(B * nn);
     float* P32 = ar.take<float>(B * nn);
     op.emax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
+    double* wre = ar.take<double>(B * n);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
     auto report = [&](const char* what, int it) -> int {
@@ -972,25 +966,70 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems, max |E| %.3e\n", what, it, tot, B, mx);
         return 0;
     };
-    // Steps 0 .. n_min-1 always run unless a correction drops below OA_TOL;
-    // after them, more run while the largest correction is at least OA_EXTEND
-    // (n_max > n_min only in the adaptive mode).
-    auto converged = [&](int it, bool& done) -> int {
-        done = false;
-        if (it + 1 >= n_max) {
-            done = true;
-            return 0;
-        }
+    // Largest correction of the last step, per problem
+    std::vector<float> pe(B);
+    auto read_emax = [&]() -> int {
         std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
         GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipStreamSynchronize(st));
-        unsigned mx = 0;
-        for (unsigned u : em) mx = std::max(mx, u);   // non-negative floats order as their bits
-        float f;
-        std::memcpy(&f, &mx, 4);
-        done = f < (it + 1 >= n_min ? OA_EXTEND : OA_TOL);
+        for (int b = 0; b < B; b++) {
+            unsigned mx = 0;   // non-negative floats order as their bits
+            for (int k = 0; k < OA_EMAX_SLOTS; k++) mx = std::max(mx, em[(size_t)b * OA_EMAX_SLOTS + k]);
+            std::memcpy(&pe[b], &mx, 4);
+        }
         return 0;
     };
+    // Refinement of X and lamv against the FP64 matrices A.  Steps 0 ..
+    // n_min-1 always run unless every correction drops below OA_TOL; after
+    // them, more run while a correction is at least oa_extend, up to n_max.
+    // With `resolve`, the problems still at or above oa_extend when it stops
+    // go to `left`; it stops early once all of them are at or above OA_GIVEUP.
+    auto refine = [&](const char* what, const double* A, double* lamv, double* axbuf, std::vector<int>& left) -> int {
+        left.clear();
+        for (int it = 0; it < n_max; it++) {
+            GMC_TRY(oa_refine(st, n, B, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
+            GMC_TRY(report(what, it));
+            const bool last = it + 1 >= n_max;
+            if (last && !resolve) break;
+            GMC_TRY(read_emax());
+            const float mx = *std::max_element(pe.begin(), pe.end());
+            if (it + 1 < n_min) {
+                if (mx < OA_TOL) break;
+                continue;
+            }
+            if (mx < oa_extend) break;
+            bool giveup = resolve != nullptr;
+            for (float e : pe)
+                if (e >= oa_extend && e < OA_GIVEUP) giveup = false;
+            if (last || giveup) {
+                if (resolve)
+                    for (int b = 0; b < B; b++)
+                        if (pe[b] >= oa_extend) left.push_back(b);
+                break;
+            }
+        }
+        return 0;
+    };
+    // The listed problems solved again in FP64: their matrices side by side
+    // in Xn (free after the refinement), eigenvectors and eigenvalues back
+    // into X and lamv.
+    auto resolve_left = [&](const char* what, const std::vector<int>& left, const double* A, double* lamv) -> int {
+        if (left.empty()) return 0;
+        const int nl = (int)left.size();
+        for (int k = 0; k < nl; k++)
+            GMC_HIP(hipMemcpyAsync(Xn + (size_t)k * nn, A + (size_t)left[k] * nn, nn * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+        GMC_TRY((*resolve)(Xn, wre, nl));
+        for (int k = 0; k < nl; k++) {
+            GMC_HIP(hipMemcpyAsync(X + (size_t)left[k] * nn, Xn + (size_t)k * nn, nn * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+            GMC_HIP(hipMemcpyAsync(lamv + (size_t)left[k] * n, wre + (size_t)k * n, n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+        }
+        if (stats) fprintf(stderr, "[gmc] OA %s: %d of %d problems solved again in FP64\n", what, nl, B);
+        return 0;
+    };
+    std::vector<int> left;
     GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
     GMC_HIP(hipMemsetAsync(out.nneg, 0, nalpha * sizeof(int), st));
     const dim3 gsq((n + 255) / 256, n, B), blk(256);
@@ -1005,13 +1044,8 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
-    for (int it = 0; it < n_max; it++) {
-        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, lam, AX, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
-        GMC_TRY(report("A+", it));
-        bool done = false;
-        GMC_TRY(converged(it, done));
-        if (done) break;
-    }
+    GMC_TRY(refine("A+", Ap, lam, AX, left));
+    GMC_TRY(resolve_left("A+", left, Ap, lam));
     // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
     hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
     GMC_HIP(hipGetLastError());
@@ -1032,13 +1066,8 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
-    for (int it = 0; it < n_max; it++) {
-        GMC_TRY(oa_refine(st, n, B, Ap, X, Xn, om2, Am, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
-        GMC_TRY(report("M", it));
-        bool done = false;
-        GMC_TRY(converged(it, done));
-        if (done) break;
-    }
+    GMC_TRY(refine("M", Ap, om2, Am, left));
+    GMC_TRY(resolve_left("M", left, Ap, om2));
     // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
     GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
     hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);

@@ -1,5 +1,7 @@
 // AC-CAS device pipeline: A+-B assembly, ERPA solve and the alpha-integrand.
 #pragma once
+#include <functional>
+
 #include "common.hpp"
 #include "eigen.hpp"
 
@@ -72,12 +74,31 @@ int integrand_from_y(hipStream_t st, int n, const double* Y, const double* eig, 
 
 // Largest number of Ogita-Aishima steps of the adaptive mixed path.
 constexpr int OA_MAX_STEPS = 6;
+// After the minimum number of steps the refinement goes on while the last
+// step's largest correction is at least OA_EXTEND, up to OA_MAX_STEPS: the
+// error left after a step scales with the square of its largest correction.
+// On 64 systems of NBasis 40-140 three steps missed 1 uEh on three, whose
+// third corrections were 9e-4 to 1.4e-2 (M); the NBasis=160 and 240 systems
+// had 9e-5 and 2.8e-4 and met it with three steps.
+constexpr float OA_EXTEND = 5e-4f;
+// A problem whose correction is still at least OA_GIVEUP after the minimum
+// steps is not converging (NBasis=400: close eigenvalue clusters of M keep it
+// near 0.1 for six steps).
+constexpr float OA_GIVEUP = 5e-2f;
+
+// FP64 eigensolve of nb problems side by side (stride n*n, overwritten by
+// their eigenvectors; w gets the eigenvalues, ascending).
+using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
 
 // Mixed path: FP32 eigen solves refined by n_min to n_max Ogita-Aishima
-// steps in FP64 (more than n_min while the corrections stay large).
+// steps in FP64 (more than n_min while the corrections stay at or above
+// oa_extend).  With `resolve` (the adaptive mode), the problems whose last
+// correction is still at or above oa_extend when the refinement stops (after
+// n_max steps, or earlier once every such problem is at or above OA_GIVEUP)
+// are solved again by `resolve` from their FP64 matrices.
 size_t erpa_mixed_bytes(int n, int batch);
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max);
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve);
 
 }  // namespace gmc

@@ -187,10 +187,24 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             else GMC_TRY(run(float(0)));
         } else {
             // GMC_MIXED: at least three steps, more (up to OA_MAX_STEPS) while
-            // the corrections stay large; GMC_MIXED2: exactly two;
-            // GMC_OA_STEPS fixes the count (diagnostics)
+            // the corrections stay large, then an FP64 solve of the problems
+            // that did not converge; GMC_MIXED2: exactly two; GMC_OA_STEPS
+            // fixes the count (diagnostics, no FP64 solves); GMC_OA_EXTEND
+            // sets the convergence threshold of the adaptive mode (0: no
+            // problem converges, every one is solved again in FP64)
             int n_min = prec == GMC_MIXED2 ? 2 : 3, n_max = prec == GMC_MIXED2 ? 2 : OA_MAX_STEPS;
             if (const char* e = getenv("GMC_OA_STEPS")) n_min = n_max = std::max(0, atoi(e));
+            float oa_extend = OA_EXTEND;
+            if (const char* e = getenv("GMC_OA_EXTEND")) oa_extend = (float)std::max(0.0, atof(e));
+            const EigSolve64 resolve = [&](double* A, double* w, int nb) -> int {
+                EigPlan<double>* p64 = nullptr;
+                GMC_TRY(st.eig_plan_for<double>(nd, nb, &p64));
+                GMC_TRY(st.resolve.reserve(p64->ws_bytes));
+                GMC_TRY(eig_bind<double>(p64, st.resolve.ptr));
+                GMC_TRY(syevd_batched<double>(s, *p64, A, w));
+                GMC_TRY(eig_check_error<double>(s, *p64));
+                return 0;
+            };
             EigPlan<float>* ep = nullptr;
             GMC_TRY(st.eig_plan_for<float>(nd, B, &ep));
             GMC_TRY(st.scratch.reserve(ep->ws_bytes));
@@ -199,7 +213,14 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             GMC_TRY(st.erpaws.reserve(align_up(mb) + outb));
             bind_out(align_up(mb));
             GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_min,
-                                           n_max));
+                                           n_max, oa_extend, n_max > n_min ? &resolve : nullptr));
+            // rare and as large as an FP64 eigensolver's workspace: not kept
+            // beyond the call that needed it
+            if (st.resolve.ptr) {
+                GMC_HIP(hipStreamSynchronize(s));
+                GMC_HIP(hipFree(st.resolve.ptr));
+                st.resolve = Workspace{};
+            }
             GMC_TRY(eig_check_error<float>(s, *ep));
         }
         GMC_HIP(hipMemcpyAsync(walpha + a0, out.W, B * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -22,6 +22,7 @@ struct State {
     Workspace sysio;       // small arrays of the system cached for host-pointer calls
     Workspace io;          // host <-> device staging of host-pointer calls
     Workspace fofo;        // the (FF|OO) and (FO|FO) integral classes of the FOFO variant
+    Workspace resolve;     // FP64 eigensolver of the mixed path's re-solves, reserved on first use
     const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
     int64_t resident_n = 0;
     double resident_probe[64] = {0};
@@ -40,7 +41,7 @@ struct State {
         eig32.clear();
         blas_release(blas);
         blas = nullptr;
-        for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo})
+        for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo, &resolve})
             if (w->ptr) (void)hipFree(w->ptr);
         scratch = Workspace{};
         resident = Workspace{};
@@ -49,6 +50,7 @@ struct State {
         sysio = Workspace{};
         io = Workspace{};
         fofo = Workspace{};
+        resolve = Workspace{};
         resident_key = nullptr;
         resident_n = 0;
         fofo_key[0] = fofo_key[1] = nullptr;

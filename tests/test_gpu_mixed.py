@@ -25,3 +25,32 @@ def test_mixed_vs_fp64(cuda, n, ni, na, ne, rank, seed, nal):
     assert e1 < 1e-6 and e2 < 1e-6
     de = np.max(np.abs(m1["eig"].cpu().numpy() - ref["eig"].cpu().numpy()))
     assert de < 1e-9
+
+
+def test_unconverged_problems_are_solved_again_in_fp64(cuda, monkeypatch, capfd):
+    """With the convergence threshold at zero no refinement converges: after
+    six steps every problem of both eigen solves is solved again in FP64, and
+    the mixed result is the FP64 path's.  At the default threshold the same
+    system converges in three steps and nothing is solved again."""
+    from gammcor_amd import host, synth
+    import accas_oracle as O
+    s = synth.make_system(48, 6, 4, 4, 144, 0.1, 0)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    a, w = host.GauLeg(0.0, 1.0, 8)
+    d = host.DeviceSystem(tw, s.occ, s.xone, s.gamma_act, 48, 6, 4, s.nele)
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64, return_eig=True)
+    monkeypatch.setenv("GMC_OA_STATS", "1")
+    capfd.readouterr()
+    m = host.ACECORR(d, a, w, prec=host.GMC_MIXED, return_eig=True)
+    err = capfd.readouterr().err
+    assert "solved again in FP64" not in err
+    assert abs(m["ecorr"] - ref["ecorr"]) < 1e-6
+    monkeypatch.setenv("GMC_OA_EXTEND", "0")
+    f = host.ACECORR(d, a, w, prec=host.GMC_MIXED, return_eig=True)
+    err = capfd.readouterr().err
+    print(err, flush=True)
+    assert "[gmc] OA A+: 8 of 8 problems solved again in FP64" in err
+    assert "[gmc] OA M: 8 of 8 problems solved again in FP64" in err
+    assert err.count("[gmc] OA M step") == 6
+    assert abs(f["ecorr"] - ref["ecorr"]) < 1e-10
+    assert np.max(np.abs(f["eig"].cpu().numpy() - ref["eig"].cpu().numpy())) < 1e-10
