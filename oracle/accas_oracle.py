@@ -16,6 +16,8 @@ Pinning (what independently fixes each function; see tests/test_oracle.py):
                   identity sum_K Y_K Y_K^T = 1/2 S (S A- S)^(-1/2) S
   aceneerpa       compiled reference ACEInteg W(alpha) values
   acecorr         compiled reference ECorr (8-point grid and ACCAS 5-point)
+  ac0cas          compiled reference AC0CAS ECorr (N = 12 .. 48); planted
+                  slips (block triangle, block structure, X/Y) fail it
 
 Deliberate difference: TwoNO1 screens |value| <= 1e-12 terms (misc.f:175,
 212, 247, 281); the restatement does not.  The seeded integrals are O(1e-2),
@@ -255,6 +257,131 @@ def acecorr(sysd, twono, alphas, weights):
     return dict(ecorr=ecorr, walpha=[r["w"] for r in res], ecasscf=res[0]["ecasscf"],
                 eig=[r["eig"] for r in res], ndimx=len(indn),
                 nneg_aplus=[r["nneg_aplus"] for r in res])
+
+
+# ---------------------------------------------------------------- AC0CAS
+
+def ac0_block_order(indn, nbasis, ninact, nact):
+    """Blocks of the alpha=0 problem in AC0CAS's order (acfd.f:2122-2395):
+    the active-active pairs (q outer, p inner), the active-inactive pairs of
+    each inactive q, the virtual-active pairs of each virtual p, then every
+    virtual-inactive pair on its own.  Returns (block label per pair,
+    position of each pair in that order), both indexed like indn."""
+    p = indn[:, 0] - 1
+    q = indn[:, 1] - 1
+    nocc = ninact + nact
+    index = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(p, q))}
+    order, label = [], np.full(len(indn), -1, dtype=np.int64)
+    nb = 0
+
+    def block(members):
+        nonlocal nb
+        members = [m for m in members if m is not None]
+        if members:
+            for m in members:
+                order.append(m)
+                label[m] = nb
+            nb += 1
+    block([index.get((ip, iq)) for iq in range(ninact, nocc) for ip in range(iq + 1, nocc)])
+    for iq in range(ninact):
+        block([index.get((ip, iq)) for ip in range(ninact, nocc)])
+    for ip in range(nocc, nbasis):
+        block([index.get((ip, iq)) for iq in range(ninact, nocc)])
+    for ip in range(nocc, nbasis):
+        for iq in range(ninact):
+            block([index.get((ip, iq))])
+    if len(order) != len(indn) or len(set(order)) != len(indn):
+        raise ValueError("AC0CAS: a selected pair lies in none of the alpha=0 blocks")
+    pos = np.empty(len(indn), dtype=np.int64)
+    pos[np.asarray(order)] = np.arange(len(order))
+    return label, pos
+
+
+def lower_in_order(a, pos):
+    """a[i, j] where pos[i] >= pos[j], mirrored: the triangle that AB0ELEMENT
+    and AB1_CAS compute in block order (acfd.f:2143, 4021-4024, 4200-4216)."""
+    keep = pos[:, None] >= pos[None, :]
+    return np.where(keep, a, a.T)
+
+
+def erpasymm0(ap, am):
+    """interpa.f:1496-1650: Y and X of the symmetric ERPA problem of one
+    block.  eig = omega with Y normalised by 2 omega^-1 Y^T A- Y = 1 and
+    X = omega^-1 A- Y when omega^2 > 1e-6; otherwise eig = omega^2 and
+    Y = X = 0."""
+    ap = 0.5 * (ap + ap.T)
+    am = 0.5 * (am + am.T)
+    lam, v = np.linalg.eigh(ap)
+    s = (v * np.sqrt(np.abs(lam))[None, :]) @ v.T
+    m = s @ (am @ s)
+    m = 0.5 * (m + m.T)
+    om2, z = np.linalg.eigh(m)
+    y = s @ z
+    x = np.zeros_like(y)
+    eig = om2.copy()
+    for k in range(len(om2)):
+        if om2[k] > SMALL_OM2:
+            eig[k] = np.sqrt(om2[k])
+            nrm = 2.0 / eig[k] * (y[:, k] @ am @ y[:, k])
+            y[:, k] *= 1.0 / np.sqrt(nrm) if nrm > 0.0 else 0.0
+            x[:, k] = am @ y[:, k] / eig[k]
+        else:
+            y[:, k] = 0.0
+    return y, x, eig
+
+
+def ac0cas(sysd, twono, indn=None):
+    """The linearised AC correction of AC0CAS (acfd.f:1902-2614).
+
+    A(alpha) = A(0) + alpha A', so A' = A(1) - A(0) is AB1_CAS's matrix
+    (acfd.f:3834).  The alpha=0 problem is solved block by block
+    (ERPASYMM0; a virtual-inactive block takes omega = A+ and
+    Y = X = 1/sqrt(2), acfd.f:2376-2395); then with the block eigenvectors
+    P = Y D Y^T, D_mn = 2 (X^T A'+ X - Y^T A'- Y)_mn / (omega_m + omega_n)
+    over nonzero omegas (acfd.f:2416-2540), and
+    ECorr = sum_IJ (C_s+C_q)(C_p+C_r) P_IJ (pr|qs) without the terms whose
+    four orbitals share a geminal (acfd.f:2548-2581)."""
+    if indn is None:
+        indn = select_pairs(sysd["occ"], sysd["nele"])
+    args = (sysd["nbasis"], sysd["ninact"], sysd["nact"], sysd["occ"], sysd["xone"], twono, sysd["gamma_act"],
+            sysd["igem"], indn)
+    a0p, a0m, ecas, ci = ab_cas(*args, 0.0)
+    a1p, a1m, _, _ = ab_cas(*args, 1.0)
+    label, pos = ac0_block_order(indn, sysd["nbasis"], sysd["ninact"], sysd["nact"])
+    n = len(indn)
+    ap0 = lower_in_order(np.asarray(a0p), pos)
+    am0 = lower_in_order(np.asarray(a0m), pos)
+    dp = lower_in_order(np.asarray(a1p) - np.asarray(a0p), pos)
+    dm = lower_in_order(np.asarray(a1m) - np.asarray(a0m), pos)
+    y = np.zeros((n, n))
+    x = np.zeros((n, n))
+    eig = np.zeros(n)
+    nocc = sysd["ninact"] + sysd["nact"]
+    p = indn[:, 0] - 1
+    q = indn[:, 1] - 1
+    for b in range(label.max() + 1):
+        m = np.nonzero(label == b)[0]
+        if len(m) == 1 and p[m[0]] >= nocc and q[m[0]] < sysd["ninact"]:
+            eig[m[0]] = ap0[m[0], m[0]]
+            y[m[0], m[0]] = x[m[0], m[0]] = 1.0 / np.sqrt(2.0)
+            continue
+        yb, xb, eb = erpasymm0(ap0[np.ix_(m, m)], am0[np.ix_(m, m)])
+        y[np.ix_(m, m)] = yb
+        x[np.ix_(m, m)] = xb
+        eig[m] = eb
+    tp = x.T @ dp @ x
+    tm = y.T @ dm @ y
+    nz = eig != 0.0
+    den = eig[:, None] + eig[None, :]
+    d = np.where(nz[:, None] & nz[None, :], 2.0 * (tp - tm) / np.where(den != 0.0, den, 1.0), 0.0)
+    pm = y @ d @ y.T
+    c = ci
+    cij = c[p] + c[q]
+    igem = np.asarray(sysd["igem"])
+    same = (igem[p] == igem[q])[:, None] & (igem[p][:, None] == igem[p][None, :]) & (igem[p] == igem[q])[None, :]
+    g = pair_integrals(twono, indn)
+    ecorr = float(np.sum(np.where(same, 0.0, cij[:, None] * cij[None, :] * pm * g)))
+    return dict(ecorr=ecorr, ecasscf=ecas, eig=eig, ndimx=n)
 
 
 def system_dict(sysm):

@@ -13,7 +13,9 @@ C       fofo  : the FOFO variant: the (FF|OO) and (FO|FO) classes of the
 C               transformed integrals written as the FFOO and FOFO record
 C               files tran4_gen produces (tran.f90:313-423), then
 C               ACEInteg_FOFO (abfofo.f90:3664) at each alpha node, printing
-C               as grid does.
+C               as grid does;
+C       ac0   : AC0CAS (acfd.f:1902), the linearised AC correction, printing
+C               its ECorr and the CASSCF energy.
 C     Pair selection for grid/dumpab mirrors accas.f:30-99 with
 C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
       Program RefDrv
@@ -160,6 +162,19 @@ C
 C
       Allocate(ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),
      $ EigVecR(NDimX*NDimX),Eig(NDimX),EGOne(NGem))
+C
+      If(Mode(1:3).Eq.'ac0') Then
+C     AC0CAS reads fock.dat for a check of its virtual-inactive blocks
+C     unless IDALTON is set (acfd.f:2365-2375, 2399-2402); the check
+C     only prints
+      IDALTON=1
+      Call AC0CAS(ECorr,ETot,TwoNO,Occ,URe,XOne,
+     $ ABPLUS,ABMIN,EigVecR,Eig,
+     $ IndN,IndX,NDimX,NBasis,NDimX,NInte1,NInte2)
+      Write(6,'(A,ES25.17)') 'AC0ECORR ',ECorr
+      Write(6,'(A,ES25.17)') 'ECASSCF ',ETot
+      Stop
+      EndIf
 C
       If(Mode(1:6).Eq.'dumpab') Then
       Call AB_CAS(ABPLUS,ABMIN,ECASSCF,URe,Occ,XOne,TwoNO,IndAux,

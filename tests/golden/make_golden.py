@@ -5,7 +5,8 @@ Needs /root/reference (the reference sources) and the reference build of
 oracle/ (``make -C oracle ref``); runs only in the build container.  For each
 case a seeded synthetic system (gammcor_amd.synth) is written in the driver's
 input format, the reference's own TwoNO1 + ACEInteg (mode grid), ACCAS (mode
-accas, its 5-point GauLeg and pair selection) or AB_CAS (mode dumpab) runs,
+accas, its 5-point GauLeg and pair selection), AB_CAS (mode dumpab) or AC0CAS
+(mode ac0, the linearised AC correction) runs,
 and the printed full-precision values are stored as JSON.  Inputs are not
 stored: they are regenerated from the seeds recorded in each fixture.
 
@@ -40,6 +41,11 @@ CASES = {
     "n48_cas44_accas": (dict(synth.CONFIGS["n48_cas44"]), "accas", 5),
     "n12_cas44_ab": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "dumpab", 1),
     "n160_cas88": (dict(synth.CONFIGS["n160_cas88"]), "grid", 8),
+    "n12_cas44_ac0": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "ac0", 1),
+    "n16_cas22_ac0": (dict(nbasis=16, ninact=3, nact=2, nelact=2, rank=30, sigma=0.1, seed=1), "ac0", 1),
+    "n20_cas44_noinact_ac0": (dict(nbasis=20, ninact=0, nact=4, nelact=4, rank=50, sigma=0.1, seed=2), "ac0", 1),
+    "n24_cas66_ac0": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "ac0", 1),
+    "n48_cas44_ac0": (dict(synth.CONFIGS["n48_cas44"]), "ac0", 1),
 }
 SMALL = [k for k in CASES if not k.startswith("n160")]
 
@@ -67,6 +73,8 @@ def parse(out):
             res["ecasscf"] = float(t[1])
         elif t[0] == "ECORR":
             res["ecorr"] = float(t[1])
+        elif t[0] == "AC0ECORR":
+            res["ac0_ecorr"] = float(t[1])
         elif line.strip().startswith("ACAlpha"):
             # ACECORR list-directed line: ACAlpha <a> W_ALPHA <w>  (acfd.f:197)
             res.setdefault("accas_alphas", []).append(float(t[1]))

@@ -286,6 +286,54 @@ def test_planted_mistakes_are_caught(monkeypatch):
     monkeypatch.setattr(O, "ab_cas", orig_ab)
 
 
+# ---------------------------------------------------------------- AC0CAS
+AC0 = ["n12_cas44_ac0", "n16_cas22_ac0", "n20_cas44_noinact_ac0", "n24_cas66_ac0", "n48_cas44_ac0"]
+
+
+@pytest.mark.parametrize("name", AC0)
+def test_ac0_matches_reference(name):
+    """The linearised AC correction against the reference's own AC0CAS
+    (acfd.f:1902) on the same system."""
+    import accas_oracle as O
+    fx = gold(name)
+    s = system_of(fx)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    r = O.ac0cas(O.system_dict(s), tw)
+    assert r["ndimx"] == fx["ndimx"]
+    assert abs(r["ecasscf"] - fx["ecasscf"]) < 1e-10
+    assert abs(r["ecorr"] - fx["ac0_ecorr"]) < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))
+
+
+def test_ac0_planted_mistakes_are_caught(monkeypatch):
+    """The other triangle of the alpha=0 blocks and of A', all pairs in one
+    alpha=0 block, and X and Y of the blocks swapped must each break the
+    match with the reference's AC0CAS.  (A(1) in place of the slope
+    A(1) - A(0) would not: in the alpha=0 eigenbasis X^T A+(0) X and
+    Y^T A-(0) Y are both diag(omega)/2, so the A(0) part cancels in D.)"""
+    import accas_oracle as O
+    fx = gold("n24_cas66_ac0")
+    s = system_of(fx)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    sd = O.system_dict(s)
+
+    def err():
+        return abs(O.ac0cas(sd, tw)["ecorr"] - fx["ac0_ecorr"])
+
+    assert err() < 1e-10
+    orig = O.lower_in_order
+    monkeypatch.setattr(O, "lower_in_order", lambda a, pos: orig(a, -pos))
+    assert err() > 1e-6
+    monkeypatch.setattr(O, "lower_in_order", orig)
+    blocks = O.ac0_block_order
+    monkeypatch.setattr(O, "ac0_block_order", lambda *a: (np.zeros(len(a[0]), dtype=np.int64), blocks(*a)[1]))
+    assert err() > 1e-6
+    monkeypatch.setattr(O, "ac0_block_order", blocks)
+    e0 = O.erpasymm0
+    monkeypatch.setattr(O, "erpasymm0", lambda ap, am: (lambda y, x, e: (x, y, e))(*e0(ap, am)))
+    assert err() > 1e-6
+    monkeypatch.setattr(O, "erpasymm0", e0)
+
+
 # ---------------------------------------------------------------- rdm2.dat
 def test_rdm2_dat_round_trip():
     from gammcor_amd import synth, host
