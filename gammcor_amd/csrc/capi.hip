@@ -12,6 +12,7 @@
 #include "gemm.hpp"
 #include "transform.hpp"
 #include "accas.hpp"
+#include "ac0.hpp"
 #include "fofo.hpp"
 
 namespace gmc {
@@ -327,6 +328,54 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
     for (int i = 0; i < nalpha; i++) e += weights[i] * w[i];   // acfd.f:199
     if (walpha)
         for (int i = 0; i < nalpha; i++) walpha[i] = w[i];
+    if (ecorr) *ecorr = e;
+    if (ecasscf) *ecasscf = r.ab.ecasscf;
+    return 0;
+}
+
+int gmc_ac0cas_dev(int nbasis, int ninact, int nact, const double* twono, const double* occ, const int* igem,
+                   const double* xone, const double* ure, const double* rdm2act, int ndimx, const int* indn,
+                   double* ecorr, double* ecasscf, void* stream)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(ndimx >= 1 && nbasis >= 2, "AC0CAS: bad sizes");
+    State& st = state();
+    hipStream_t s = dev_stream(stream);
+    st.sys_valid = false;   // acws is overwritten: drop the host-call system cache
+    AcRun r;
+    GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, r));
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    std::vector<int> hind(2 * (size_t)ndimx);
+    GMC_HIP(hipMemcpyAsync(hind.data(), indn, hind.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    Ac0Blocks b;
+    GMC_TRY(ac0_blocks(nbasis, ninact, nact, ndimx, hind.data(), b));
+    // the ERPA blocks: one FP64 problem of order n1 at alpha = 0
+    const int n1 = (int)b.rows1.size();
+    ErpaPlan<double> pl;
+    size_t ews = 0;
+    if (n1) {
+        EigPlan<double>* ep = nullptr;
+        GMC_TRY(st.eig_plan_for<double>(n1, 1, &ep));
+        GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+        GMC_TRY(eig_bind<double>(ep, st.scratch.ptr));
+        GMC_TRY(erpa_plan<double>(n1, 1, &pl));
+        pl.eig = ep;
+        ews = align_up(pl.ws_bytes);
+    }
+    const size_t n1e = (size_t)std::max(n1, 1) * sizeof(double);
+    const size_t outb = carve_bytes({sizeof(double), n1e, sizeof(int)});
+    const size_t acb = ac0_bytes(ndimx, n1);
+    GMC_TRY(st.erpaws.reserve(ews + outb + acb));
+    char* base = static_cast<char*>(st.erpaws.ptr);
+    ErpaOut out;
+    out.W = reinterpret_cast<double*>(base + ews);
+    out.eig = reinterpret_cast<double*>(base + ews + align_up(sizeof(double)));
+    out.nneg = reinterpret_cast<int*>(base + ews + align_up(sizeof(double)) + align_up(n1e));
+    double e = 0.0;
+    GMC_TRY(ac0_correction(s, r.ab, r.Gw, ndimx, b, &pl, base, out, base + ews + outb, acb, &e));
+    if (n1) GMC_TRY(eig_check_error<double>(s, *pl.eig));
     if (ecorr) *ecorr = e;
     if (ecasscf) *ecasscf = r.ab.ecasscf;
     return 0;

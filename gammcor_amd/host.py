@@ -141,6 +141,18 @@ def ACECORR(dsys, alphas, weights, prec=GMC_FP64, return_eig=False, stream=None,
     return out
 
 
+def AC0CAS(dsys, stream=None):
+    """The linearised AC correction (AC0CAS, acfd.f:1902-2614), FP64: the
+    alpha=0 problem in AC0CAS's blocks and the first-order term with
+    A' = A(1) - A(0).  Returns a dict with ecorr and ecasscf (w/o ENuc)."""
+    ec, es = np.zeros(1), np.zeros(1)
+    ure = dsys.ure.data_ptr() if dsys.ure is not None else None
+    _lib.call("gmc_ac0cas_dev", dsys.nbasis, dsys.ninact, dsys.nact, dsys.twono.data_ptr(), dsys.occ.data_ptr(),
+              dsys.igem.data_ptr(), dsys.xone.data_ptr(), ure, dsys.rdm2act.data_ptr(), dsys.ndimx,
+              dsys.indn.data_ptr(), ec.ctypes.data, es.ctypes.data, stream)
+    return dict(ecorr=float(ec[0]), ecasscf=float(es[0]))
+
+
 def ACECORR_FOFO(ao, uaomo, dsys, alphas, weights, prec=GMC_FP64, alpha_batch=0, return_eig=False, stream=None):
     """The FOFO variant (ACEInteg_FOFO, abfofo.f90:3664): packed AO integrals
     ``ao`` and orbitals ``uaomo`` (URe(NO,AO), column-major, both cuda

@@ -144,6 +144,19 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
                     int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights,
                     double* ecorr, double* walpha, double* ecasscf, double* eig, int* nneg, void* stream);
 
+/* AC0CAS (acfd.f:1902-2614): the linearised AC correction, FP64.  The
+ * alpha=0 problem is solved in AC0CAS's blocks (active-active, active-inactive
+ * per inactive orbital, virtual-active per virtual orbital; a virtual-inactive
+ * pair alone), A' = A(1) - A(0) is AB1_CAS's matrix (acfd.f:3834), and
+ * ECorr = sum_IJ (C_s+C_q)(C_p+C_r) P_IJ (pr|qs) over the pairs not all in
+ * one geminal, P = Y D Y^T (acfd.f:2437-2592).  Arguments as gmc_acecorr_dev
+ * (device arrays); outputs ecorr and ecasscf (w/o ENuc) are host scalars.
+ * Like the reference, the virtual-inactive blocks take omega = A+ without
+ * reading fock.dat. */
+int gmc_ac0cas_dev(int nbasis, int ninact, int nact, const double* twono, const double* occ, const int* igem,
+                   const double* xone, const double* ure, const double* rdm2act, int ndimx, const int* indn,
+                   double* ecorr, double* ecasscf, void* stream);
+
 /* The FOFO variant of ACECORR (ACEInteg_FOFO, abfofo.f90:3664-3789, with
  * AB_CAS_FOFO 8-293 and ACEneERPA_FOFO 3791-3927): the integrals enter as
  * packed AO integrals ao (NAddr3 order, device) with the AO->NO orbitals

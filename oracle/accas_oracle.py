@@ -262,7 +262,7 @@ def acecorr(sysd, twono, alphas, weights):
 # ---------------------------------------------------------------- AC0CAS
 
 def ac0_block_order(indn, nbasis, ninact, nact):
-    """Blocks of the alpha=0 problem in AC0CAS's order (acfd.f:2122-2395):
+    """Blocks of the alpha=0 problem in AC0CAS's order (acfd.f:2122-2416):
     the active-active pairs (q outer, p inner), the active-inactive pairs of
     each inactive q, the virtual-active pairs of each virtual p, then every
     virtual-inactive pair on its own.  Returns (block label per pair,
@@ -299,7 +299,7 @@ def ac0_block_order(indn, nbasis, ninact, nact):
 
 def lower_in_order(a, pos):
     """a[i, j] where pos[i] >= pos[j], mirrored: the triangle that AB0ELEMENT
-    and AB1_CAS compute in block order (acfd.f:2143, 4021-4024, 4200-4216)."""
+    and AB1_CAS compute in block order (acfd.f:2168, 2244, 2323, 4200-4216)."""
     keep = pos[:, None] >= pos[None, :]
     return np.where(keep, a, a.T)
 
@@ -336,11 +336,11 @@ def ac0cas(sysd, twono, indn=None):
     A(alpha) = A(0) + alpha A', so A' = A(1) - A(0) is AB1_CAS's matrix
     (acfd.f:3834).  The alpha=0 problem is solved block by block
     (ERPASYMM0; a virtual-inactive block takes omega = A+ and
-    Y = X = 1/sqrt(2), acfd.f:2376-2395); then with the block eigenvectors
+    Y = X = 1/sqrt(2), acfd.f:2362-2416); then with the block eigenvectors
     P = Y D Y^T, D_mn = 2 (X^T A'+ X - Y^T A'- Y)_mn / (omega_m + omega_n)
-    over nonzero omegas (acfd.f:2416-2540), and
+    over nonzero omegas (acfd.f:2437-2556), and
     ECorr = sum_IJ (C_s+C_q)(C_p+C_r) P_IJ (pr|qs) without the terms whose
-    four orbitals share a geminal (acfd.f:2548-2581)."""
+    four orbitals share a geminal (acfd.f:2557-2592)."""
     if indn is None:
         indn = select_pairs(sysd["occ"], sysd["nele"])
     args = (sysd["nbasis"], sysd["ninact"], sysd["nact"], sysd["occ"], sysd["xone"], twono, sysd["gamma_act"],
