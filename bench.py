@@ -113,7 +113,7 @@ def main():
     ap.add_argument("--config", default="n240_cas88")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip", default="",
-                    help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in,fofo")
+                    help="comma list of comparison modes to skip: fp64_batched,fp64_drop_in,fofo,ac0")
     ap.add_argument("--path", choices=("auto", "incore", "fofo"), default="auto",
                     help="integrals: incore = TwoNO1 then the packed NO integrals (the reference's ACECORR), "
                          "fofo = only (FF|OO) and (FO|FO) (its FOFO variant); auto = fofo from NBasis 320 on, "
@@ -199,6 +199,19 @@ def main():
         rf = fofo_solve()
         torch.cuda.synchronize()
         results["fofo_mixed"] = dict(sec=time.perf_counter() - t0, ecorr=rf["ecorr"])
+    if "ac0" not in args.skip and path == "incore":
+        # the linearised AC correction (AC0CAS) of the same system, FP64: the
+        # transform, the assembly, the alpha=0 blocks and the first-order term
+        def ac0_solve():
+            work.copy_(ao)
+            _lib.call("gmc_twono1_dev", work.data_ptr(), ure.data_ptr(), dsys.nbasis, work.numel(), stream)
+            return host.AC0CAS(dsys, stream=stream)
+        ac0_solve()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ra = ac0_solve()
+        torch.cuda.synchronize()
+        results["ac0_fp64"] = dict(sec=time.perf_counter() - t0, ecorr=ra["ecorr"])
     mixed = results["mixed"]
     fp64 = results.get("fp64_drop_in") or results.get("fp64_batched")
     ref64 = results.get("fp64_batched") or fp64
@@ -253,6 +266,8 @@ def main():
         "fofo_mixed_solves_per_s": (1.0 / results["fofo_mixed"]["sec"]) if "fofo_mixed" in results else None,
         "fofo_mixed_abs_decorr": abs(results["fofo_mixed"]["ecorr"] - ref64["ecorr"]) if ("fofo_mixed" in results
                                                                                         and ref64) else None,
+        "ac0_fp64_solves_per_s": (1.0 / results["ac0_fp64"]["sec"]) if "ac0_fp64" in results else None,
+        "ecorr_ac0": results["ac0_fp64"]["ecorr"] if "ac0_fp64" in results else None,
         "ecorr_fp64": ref64["ecorr"] if ref64 else None,
         "ecorr_mixed": mixed["ecorr"],
         "abs_decorr": abs(mixed["ecorr"] - ref64["ecorr"]) if ref64 else None,
