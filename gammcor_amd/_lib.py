@@ -20,6 +20,7 @@ SIGNATURES = {
     "gmc_finalize": [],
     "gmc_twono1": [_vp, _vp, _c_int, _c_i64],
     "gmc_twono1_dev": [_vp, _vp, _c_int, _c_i64, _vp],
+    "gmc_ac0cas": [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int],
     "gmc_ac0cas_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp],
     "gmc_acecorr_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp,
                         _vp, _vp, _vp, _vp, _vp, _vp, _vp],

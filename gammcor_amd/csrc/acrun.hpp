@@ -26,4 +26,7 @@ int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* tw
 int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, const double* alphas, double* walpha,
                      double* eig_dev, int* nneg_host);
 
+// AC0CAS (acfd.f:1902) for a prepared system; indn_host is IndN(2, ndimx).
+int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr);
+
 }  // namespace gmc

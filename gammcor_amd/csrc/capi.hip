@@ -236,6 +236,40 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
     return 0;
 }
 
+// AC0CAS for a prepared system (A(0), A(1) and Gw assembled in r).
+int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr)
+{
+    State& st = state();
+    const int ndimx = r.sys.ndimx;
+    Ac0Blocks b;
+    GMC_TRY(ac0_blocks(r.sys.N, r.sys.nI, r.sys.nA, ndimx, indn_host, b));
+    // the ERPA blocks: one FP64 problem of order n1 at alpha = 0
+    const int n1 = (int)b.rows1.size();
+    ErpaPlan<double> pl;
+    size_t ews = 0;
+    if (n1) {
+        EigPlan<double>* ep = nullptr;
+        GMC_TRY(st.eig_plan_for<double>(n1, 1, &ep));
+        GMC_TRY(st.scratch.reserve(ep->ws_bytes));
+        GMC_TRY(eig_bind<double>(ep, st.scratch.ptr));
+        GMC_TRY(erpa_plan<double>(n1, 1, &pl));
+        pl.eig = ep;
+        ews = align_up(pl.ws_bytes);
+    }
+    const size_t n1e = (size_t)std::max(n1, 1) * sizeof(double);
+    const size_t outb = carve_bytes({sizeof(double), n1e, sizeof(int)});
+    const size_t acb = ac0_bytes(ndimx, n1);
+    GMC_TRY(st.erpaws.reserve(ews + outb + acb));
+    char* base = static_cast<char*>(st.erpaws.ptr);
+    ErpaOut out;
+    out.W = reinterpret_cast<double*>(base + ews);
+    out.eig = reinterpret_cast<double*>(base + ews + align_up(sizeof(double)));
+    out.nneg = reinterpret_cast<int*>(base + ews + align_up(sizeof(double)) + align_up(n1e));
+    GMC_TRY(ac0_correction(s, r.ab, r.Gw, ndimx, b, &pl, base, out, base + ews + outb, acb, ecorr));
+    if (n1) GMC_TRY(eig_check_error<double>(s, *pl.eig));
+    return 0;
+}
+
 }  // namespace gmc
 
 template <class T>
@@ -349,33 +383,8 @@ int gmc_ac0cas_dev(int nbasis, int ninact, int nact, const double* twono, const 
     std::vector<int> hind(2 * (size_t)ndimx);
     GMC_HIP(hipMemcpyAsync(hind.data(), indn, hind.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     GMC_HIP(hipStreamSynchronize(s));
-    Ac0Blocks b;
-    GMC_TRY(ac0_blocks(nbasis, ninact, nact, ndimx, hind.data(), b));
-    // the ERPA blocks: one FP64 problem of order n1 at alpha = 0
-    const int n1 = (int)b.rows1.size();
-    ErpaPlan<double> pl;
-    size_t ews = 0;
-    if (n1) {
-        EigPlan<double>* ep = nullptr;
-        GMC_TRY(st.eig_plan_for<double>(n1, 1, &ep));
-        GMC_TRY(st.scratch.reserve(ep->ws_bytes));
-        GMC_TRY(eig_bind<double>(ep, st.scratch.ptr));
-        GMC_TRY(erpa_plan<double>(n1, 1, &pl));
-        pl.eig = ep;
-        ews = align_up(pl.ws_bytes);
-    }
-    const size_t n1e = (size_t)std::max(n1, 1) * sizeof(double);
-    const size_t outb = carve_bytes({sizeof(double), n1e, sizeof(int)});
-    const size_t acb = ac0_bytes(ndimx, n1);
-    GMC_TRY(st.erpaws.reserve(ews + outb + acb));
-    char* base = static_cast<char*>(st.erpaws.ptr);
-    ErpaOut out;
-    out.W = reinterpret_cast<double*>(base + ews);
-    out.eig = reinterpret_cast<double*>(base + ews + align_up(sizeof(double)));
-    out.nneg = reinterpret_cast<int*>(base + ews + align_up(sizeof(double)) + align_up(n1e));
     double e = 0.0;
-    GMC_TRY(ac0_correction(s, r.ab, r.Gw, ndimx, b, &pl, base, out, base + ews + outb, acb, &e));
-    if (n1) GMC_TRY(eig_check_error<double>(s, *pl.eig));
+    GMC_TRY(ac0_run(s, r, hind.data(), &e));
     if (ecorr) *ecorr = e;
     if (ecasscf) *ecasscf = r.ab.ecasscf;
     return 0;

@@ -471,6 +471,20 @@ int gmc_aceinteg_fofo(int prec, double* ecorr, double* egone, const double* jint
     return aceinteg_point(s, r, prec, ndimx, acalpha, ecorr, egone, abplus, abmin, eigvecr, eig);
 }
 
+int gmc_ac0cas(double* ecorr, double* etot, const double* twono, int64_t ninte2, const double* ure, const double* occ,
+               const double* xone, const int* igem, const double* rdm2act, const int* indn, int ndimx, int nbasis,
+               int ninact, int nact)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(ndimx >= 1 && nbasis >= 2, "AC0CAS: bad sizes");
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system(s, nbasis, ninact, nact, ndimx, twono, ninte2, occ, igem, xone, ure, rdm2act, indn, &r));
+    GMC_TRY(ac0_run(s, *r, indn, ecorr));
+    if (etot) *etot = r->ab.ecasscf;
+    return 0;
+}
+
 int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono, int64_t ninte2,
                 const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
                 int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,

@@ -140,6 +140,39 @@ C     CICoef as AB_CAS leaves it in /APSG/: sqrt(n), negative for n < 1/2
       Return
       End
 C
+C     AC0CAS (acfd.f:1902-2614), the linearised AC correction, on the GPU.
+C     Inputs as the reference takes them (COMMON /CAS/ and /APSG/, rdm2.dat);
+C     ECorr and ETot come back, the scratch arrays ABPLUS, ABMIN, EigY and
+C     Eig are not written.  fock.dat is not read: the reference only checks
+C     the virtual-inactive blocks against it.
+      Subroutine AC0CAS(ECorr,ETot,TwoNO,Occ,URe,XOne,
+     $ ABPLUS,ABMIN,EigY,Eig,
+     $ IndN,IndX,NDimX,NBasis,NDim,NInte1,NInte2)
+      use gammcor_hip
+      Implicit Real*8 (A-H,O-Z)
+      Include 'commons.inc'
+      Dimension URe(NBasis,NBasis),XOne(NInte1),Occ(NBasis),
+     $ TwoNO(NInte2),ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),
+     $ Eig(NDimX),EigY(NDimX*NDimX),IndX(NDim),IndN(2,NDim)
+      Real*8, Allocatable :: RDM2Act(:)
+      If(NoSt.Ne.1) Then
+      Write(6,'(1X,A)') 'gammcor_hip: AC0CAS supports NoSt=1'
+      Stop 1
+      EndIf
+      Call GmcCheckIndX(IndX,NDimX,NDim,'AC0CAS')
+      Write(6,'(1X,A)') 'gammcor_hip: AC0CAS on the GPU'
+      Allocate(RDM2Act(Max(1,NAcCAS**2*(NAcCAS**2+1)/2)))
+      Call GmcRdm2Act(RDM2Act,NAcCAS)
+      Call gmc_check(gmc_ac0cas(ECorr,ETot,TwoNO,Int(NInte2,8),URe,Occ,
+     $ XOne,IGem,RDM2Act,IndN,NDimX,NBasis,NInAcCAS,NAcCAS),'AC0CAS')
+      Deallocate(RDM2Act)
+C     CICoef as AC0CAS leaves it in /APSG/ (acfd.f:2015-2020)
+      Do I=1,NBasis
+      CICoef(I)=Sign(Sqrt(Occ(I)),Occ(I)-0.5D0)
+      EndDo
+      Return
+      End
+C
 C     A whole file of fixed-length records (the reference's direct-access
 C     FFOO and FOFO files carry no record markers) into X(1:N).
       Subroutine GmcReadRecords(FName,X,N)

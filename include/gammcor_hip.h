@@ -77,6 +77,15 @@ int gmc_erpasymm1(double* eigvecr, double* eig, double* aplsqrt, double* abmin, 
 int gmc_aceneerpa(double* ecorr, const double* eigvecr, const double* eig, const double* twono, int64_t ninte2,
                   const double* occ, const int* igem, const double* cicoef, const int* indn, int nbasis, int ndimx);
 
+/* AC0CAS(ECorr,ETot,TwoNO,Occ,URe,XOne,ABPLUS,ABMIN,EigY,Eig,IndN,IndX,
+ * NDimX,NBasis,NDim,NInte1,NInte2) - acfd.f:1902-1904: the linearised AC
+ * correction, FP64 (see gmc_ac0cas_dev).  Host arrays as gmc_aceinteg takes
+ * them; ecorr = ECorr, etot = ETot (CASSCF energy w/o ENuc).  The reference's
+ * scratch arrays ABPLUS, ABMIN, EigY and Eig are not written. */
+int gmc_ac0cas(double* ecorr, double* etot, const double* twono, int64_t ninte2, const double* ure, const double* occ,
+               const double* xone, const int* igem, const double* rdm2act, const int* indn, int ndimx, int nbasis,
+               int ninact, int nact);
+
 /* ACEInteg(ECorr,TwoNO,URe,Occ,XOne,UNOAO,ABPLUS,ABMIN,EigVecR,Eig,EGOne,
  * NGOcc,Title,NBasis,NInte1,NInte2,NDim,NGem,IndAux,ACAlpha,IndN,IndX,NDimX)
  * - acfd.f:4832-4836, its CASSCF branch with NoSt = 1: AB_CAS -> ERPASYMM1 ->

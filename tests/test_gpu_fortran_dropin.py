@@ -1,5 +1,5 @@
 """The reference program with the Fortran drop-in layer linked in place of its
-own TwoNO1, AB_CAS, ERPASYMM1, ACEneERPA, ACEInteg and ACEInteg_FOFO
+own TwoNO1, AB_CAS, ERPASYMM1, ACEneERPA, ACEInteg, ACEInteg_FOFO and AC0CAS
 (oracle/_ref/ref_driver_gpu, built by `make -C oracle ref_gpu` from the
 reference sources plus gammcor_amd/fortran): the reference's own ACECORR and
 driver run unchanged and call the GPU through iso_c_binding.  Its printed
@@ -78,4 +78,15 @@ def test_reference_fofo_driver_through_the_fortran_drop_in(cuda, name):
     assert "gammcor_hip: ACEInteg_FOFO on the GPU" in out
     assert np.max(np.abs(np.array(res["walpha"]) - np.array(fx["walpha"]))) < 1e-9
     assert abs(res["ecorr"] - fx["ecorr"]) < 1e-9
+    assert abs(res["ecasscf"] - fx["ecasscf"]) < 1e-10
+
+
+@needs_driver
+@pytest.mark.parametrize("name", ["n16_cas22_ac0", "n20_cas44_noinact_ac0", "n48_cas44_ac0"])
+def test_reference_ac0_driver_through_the_fortran_drop_in(cuda, name):
+    """The driver's AC0CAS call (with its TwoNO1) lands on the drop-in, which
+    reads COMMON and rdm2.dat as the reference does and runs on the GPU."""
+    fx, res, out = _run(name, "ac0")
+    assert "gammcor_hip: AC0CAS on the GPU" in out
+    assert abs(res["ac0_ecorr"] - fx["ac0_ecorr"]) < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))
     assert abs(res["ecasscf"] - fx["ecasscf"]) < 1e-10
