@@ -15,12 +15,14 @@ C               files tran4_gen produces (tran.f90:313-423), then
 C               ACEInteg_FOFO (abfofo.f90:3664) at each alpha node, printing
 C               as grid does;
 C       ac0   : AC0CAS (acfd.f:1902), the linearised AC correction, printing
-C               its ECorr and the CASSCF energy.
+C               its ECorr and the CASSCF energy;
+C       ac0fofo: its FOFO version AC0CAS_FOFO (abfofo.f90:1485) on the FFOO
+C               and FOFO record files written as fofo does.
 C     Pair selection for grid/dumpab mirrors accas.f:30-99 with
 C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
       Program RefDrv
       use types
-      use abfofo, only : ACEInteg_FOFO
+      use abfofo, only : ACEInteg_FOFO, AC0CAS_FOFO
       Implicit Real*8 (A-H,O-Z)
       Include 'commons.inc'
       Character*60 Title
@@ -163,7 +165,7 @@ C
       Allocate(ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),
      $ EigVecR(NDimX*NDimX),Eig(NDimX),EGOne(NGem))
 C
-      If(Mode(1:3).Eq.'ac0') Then
+      If(Trim(Mode).Eq.'ac0') Then
 C     AC0CAS reads fock.dat for a check of its virtual-inactive blocks
 C     unless IDALTON is set (acfd.f:2365-2375, 2399-2402); the check
 C     only prints
@@ -189,7 +191,7 @@ C
       Stop
       EndIf
 C
-      If(Mode(1:4).Eq.'fofo') Then
+      If(Mode(1:4).Eq.'fofo'.Or.Mode(1:7).Eq.'ac0fofo') Then
       NOcc=NInAc+NAct
       Open(14,File='FFOO',Status='REPLACE',Access='DIRECT',
      $ Form='UNFORMATTED',Recl=8*NBasis*NBasis)
@@ -209,6 +211,15 @@ C
       EndDo
       EndDo
       Close(14)
+      EndIf
+C
+      If(Mode(1:7).Eq.'ac0fofo') Then
+      Call AC0CAS_FOFO(ECorr,ETot,Occ,URe,XOne,ABPLUS,ABMIN,
+     $ IndN,IndX,IGem,NAct,NInAc,NDimX,NBasis,NDimX,NInte1,
+     $ NoSt,'FFOO','FOFO')
+      Write(6,'(A,ES25.17)') 'AC0ECORR ',ECorr
+      Write(6,'(A,ES25.17)') 'ECASSCF ',ETot
+      Stop
       EndIf
 C
       ECorr=0.D0
