@@ -178,7 +178,13 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     constexpr int VN = 16 / sizeof(T);
     extern __shared__ char smem[];
     const int n = g.n, k = g.k, kb = g.kb, nb = g.nb, G = g.G;
-    const int p = blockIdx.x / G, gi = blockIdx.x % G;
+    // Workgroups are dispatched round-robin over the 8 XCDs.  When G divides
+    // the workgroups per XCD, number them XCD by XCD so the G workgroups of a
+    // problem share one XCD (and its L2) for their barriers and partial sums.
+    const int nwg = (int)gridDim.x;
+    const int lin = (nwg % 8 == 0 && (nwg / 8) % G == 0) ? (int)(blockIdx.x & 7) * (nwg / 8) + (int)(blockIdx.x >> 3)
+                                                      : (int)blockIdx.x;
+    const int p = lin / G, gi = lin % G;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nunit = (n + SY_OB - 1) / SY_OB;
     const int nslots = owned_slots(nunit, gi, G);
