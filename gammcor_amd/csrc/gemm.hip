@@ -17,6 +17,14 @@ static int launch(hipStream_t s, const GemmShape& g, T alpha, const void* A, con
     // algorithmic flops: 2mnk per product (half of it when only the lower triangle is produced)
     const double flops = 2.0 * g.m * g.n * (double)g.k * g.batch * (g.uplo == 1 ? 0.5 : 1.0);
     ProfScope ps(s, sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", flops);
+    if (!PTR && g.uplo == 1 && g.m == g.n) {
+        const int nt = (g.m + BM - 1) / BM, ntl = nt * (nt + 1) / 2;
+        GMC_CHECK((int64_t)ntl * g.batch < (1ll << 31), "gemm: %d x %d lower tiles", ntl, g.batch);
+        hipLaunchKernelGGL((gemm_lower_kernel<T, BK>), dim3(ntl * g.batch), dim3(256), 0, s, g, alpha,
+                           static_cast<const T*>(A), static_cast<const T*>(B), beta, static_cast<T*>(C), ntl);
+        GMC_HIP(hipGetLastError());
+        return 0;
+    }
     hipLaunchKernelGGL((gemm_kernel<T, BK, PTR>), grid, dim3(256), 0, s, g, alpha, A, B, beta, C);
     GMC_HIP(hipGetLastError());
     return 0;

@@ -165,6 +165,23 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmShape g, T alpha, const v
     gemm_tile<T, BK>(g, bi, bj, alpha, A, B, beta, C);
 }
 
+// The lower-triangle form of a square product (uplo == 1, m == n): one
+// workgroup per tile on or below the diagonal (ntl per problem), no empty
+// workgroups for the upper tiles.  Workgroups are dispatched round-robin over
+// the 8 XCDs; they are numbered XCD by XCD so a problem's tiles, which share
+// its A and B panels, sit on as few XCDs (L2s) as possible.
+template <class T, int BK>
+__global__ __launch_bounds__(256) void gemm_lower_kernel(GemmShape g, T alpha, const T* A0, const T* B0, T beta,
+                                                          T* C0, int ntl)
+{
+    const int nwg = (int)gridDim.x;
+    const int lin = nwg % 8 == 0 ? (int)(blockIdx.x & 7) * (nwg / 8) + (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+    const int b = lin / ntl;
+    int ti, tj;
+    lower_tile(lin - b * ntl, ti, tj);
+    gemm_tile<T, BK>(g, ti * BM, tj * BN, alpha, A0 + (int64_t)b * g.sa, B0 + (int64_t)b * g.sb, beta,
+                     C0 + (int64_t)b * g.sc);
+}
 
 template <class T> struct GemmBK { static constexpr int value = sizeof(T) == 8 ? 16 : 32; };
 
