@@ -146,11 +146,11 @@ def main():
     stream = torch.cuda.current_stream().cuda_stream
 
     # peaks of one MI355X (datasheet): HBM3E 8 TB/s; dense FP64 MFMA 78.6 TF/s,
-    # FP32 MFMA 157.3 TF/s
-    PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12}
+    # FP32 MFMA 157.3 TF/s, FP16 MFMA 2.5 PF/s
+    PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12, "mfma_f16": 2.5e15}
     KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
                "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
-               "ab_cas_main": ("hbm", PEAK["hbm"])}
+               "ab_cas_main": ("hbm", PEAK["hbm"]), "blas_gemm_f16": ("mfma", PEAK["mfma_f16"])}
     PHASES = [f"eig_{ph}_{t}" for ph in ("tridiag", "dc", "backtr") for t in ("f32", "f64")] + ["twono1"]
     for ab in ("nn", "nt", "tn", "tt"):     # rocBLAS products, one kernel per transpose pair
         KERNELS[f"blas_gemm_f64_{ab}"] = ("mfma", PEAK["mfma_f64"])
@@ -254,7 +254,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 eigen solves + f64 (Ogita-Aishima refinement, transform, assembly, quadrature)",
+        "dtype": "f32 eigen solves + f64 (Ogita-Aishima refinement, transform, assembly, quadrature); "
+                 "refinement updates with corrections >= 5e-3 from f16 inputs (f32 accumulation)",
         "data": "synthetic (seeded Cholesky-factor AO integrals, QR orbitals, random-CI RDMs)",
         "config": {"workload": f"AC-CAS {args.config}: NBasis={n}, {s.ninact} inactive, CAS({s.nelact}e,{s.nact}o), "
                                f"NDimX={dsys.ndimx}, {nalpha} Gauss-Legendre alpha points batched, rank-{cfg['rank']} AO",

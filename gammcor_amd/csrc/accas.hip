@@ -12,6 +12,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_fp16.h>
+
 #include "accas.hpp"
 #include "gemm.hpp"
 #include "prof.hpp"
@@ -857,13 +859,22 @@ __global__ void oa_add_kernel(const double* X, const float* P, double* Xn, int64
         Xn[i] = X[i] + (double)P[i];
 }
 
-// The update X E is formed in FP32: every |E_ij| <= 1/8 and the product is a
-// correction of X, so its FP32 rounding (~1e-7 |X||E|) stays far below what
-// the step removes and the next step corrects it; the residual terms
-// (A X, X^T A X, X^T X) are FP64.
+// X and E / scale in IEEE half precision (|X_ij| <= 1; E scaled to |E| <= ~1).
+__global__ void f64_to_f16_kernel(const double* a, __half* b, int64_t count, double scale)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = __float2half((float)(a[i] * scale));   // via float: 2^-24 against half's 2^-11
+}
+
+// The update X E is a correction of X: its rounding adds about u |X||E| to
+// an error the step leaves at about max|E|^2 (the step converges
+// quadratically).  A step with max|E| >= half_from forms it from IEEE half
+// inputs (u = 4.9e-4) with FP32 accumulation, which keeps u max|E| at or
+// below a tenth of max|E|^2; smaller corrections use FP32 (u = 6e-8).  The
+// residual terms (A X, X^T A X, X^T X) are FP64.
 static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
-                     float* E32, float* P32)
+                     float* E32, float* P32, float half_from)
 {
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
@@ -878,11 +889,31 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
                        op.pairs, op.rot, op.emax);
     GMC_HIP(hipGetLastError());
+    // the largest off-diagonal correction of the batch (non-negative floats
+    // order as their bits)
+    std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
+    GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipStreamSynchronize(st));
+    unsigned mxb = 0;
+    for (unsigned u : em) mxb = std::max(mxb, u);
+    float emax;
+    std::memcpy(&emax, &mxb, 4);
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, X, X32, (int64_t)B * nn);
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, AX, E32, (int64_t)B * nn);
-    GMC_HIP(hipGetLastError());
-    GMC_TRY(gemm_big<float>(st, gnn, 1.0f, X32, E32, 0.0f, P32));
+    if (emax >= half_from && emax > 0.f) {
+        // half inputs in the FP32 buffers' space, E scaled by 1 / emax
+        __half* X16 = reinterpret_cast<__half*>(X32);
+        __half* E16 = reinterpret_cast<__half*>(E32);
+        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, X, X16, (int64_t)B * nn, 1.0);
+        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, AX, E16, (int64_t)B * nn,
+                           1.0 / (double)emax);
+        GMC_HIP(hipGetLastError());
+        GMC_TRY(gemm_f16(st, gnn, emax, X16, E16, 0.0f, P32));
+    } else {
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, X, X32, (int64_t)B * nn);
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, AX, E32, (int64_t)B * nn);
+        GMC_HIP(hipGetLastError());
+        GMC_TRY(gemm_big<float>(st, gnn, 1.0f, X32, E32, 0.0f, P32));
+    }
     hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
@@ -916,7 +947,7 @@ size_t erpa_mixed_bytes(int n, int batch)
 // the integrand in FP64.
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve)
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from)
 {
     GMC_CHECK(ep.bound && ep.batch == nalpha && ep.n == n, "ERPA mixed: eigensolver plan mismatch");
     const int B = nalpha;
@@ -987,7 +1018,8 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     auto refine = [&](const char* what, const double* A, double* lamv, double* axbuf, std::vector<int>& left) -> int {
         left.clear();
         for (int it = 0; it < n_max; it++) {
-            GMC_TRY(oa_refine(st, n, B, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32, P32));
+            GMC_TRY(oa_refine(st, n, B, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32, P32,
+                              half_from));
             GMC_TRY(report(what, it));
             const bool last = it + 1 >= n_max;
             if (last && !resolve) break;

@@ -86,6 +86,12 @@ constexpr float OA_EXTEND = 5e-4f;
 // near 0.1 for six steps).
 constexpr float OA_GIVEUP = 5e-2f;
 
+// A refinement step whose largest correction is at least OA_HALF forms its
+// update X E from IEEE half inputs (FP32 accumulation): the half rounding,
+// about 4.9e-4 max|E|, stays at or below a tenth of the max|E|^2 the step
+// leaves.
+constexpr float OA_HALF = 5e-3f;
+
 // FP64 eigensolve of nb problems side by side (stride n*n, overwritten by
 // their eigenvectors; w gets the eigenvalues, ascending).
 using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
@@ -95,10 +101,12 @@ using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
 // oa_extend).  With `resolve` (the adaptive mode), the problems whose last
 // correction is still at or above oa_extend when the refinement stops (after
 // n_max steps, or earlier once every such problem is at or above OA_GIVEUP)
-// are solved again by `resolve` from their FP64 matrices.
+// are solved again by `resolve` from their FP64 matrices.  Steps with a
+// largest correction of at least half_from form the update in half
+// precision (OA_HALF).
 size_t erpa_mixed_bytes(int n, int batch);
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve);
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from);
 
 }  // namespace gmc

@@ -93,6 +93,25 @@ int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B,
     return 0;
 }
 
+int gemm_f16(hipStream_t s, const GemmShape& g, float alpha, const void* A, const void* B, float beta, float* C)
+{
+    if (g.m <= 0 || g.n <= 0 || g.batch <= 0) return 0;
+    GMC_CHECK(g.uplo == 0, "gemm_f16: full products only");
+    rocblas_handle h = nullptr;
+    GMC_TRY(blas_handle(&h));
+    GMC_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+    const rocblas_operation ta = g.ta ? rocblas_operation_transpose : rocblas_operation_none;
+    const rocblas_operation tb = g.tb ? rocblas_operation_transpose : rocblas_operation_none;
+    ProfScope ps(s, "blas_gemm_f16", 2.0 * g.m * g.n * (double)g.k * g.batch);
+    const rocblas_status rs = rocblas_gemm_strided_batched_ex(
+        h, ta, tb, g.m, g.n, g.k, &alpha, A, rocblas_datatype_f16_r, (rocblas_int)g.lda, g.sa, B,
+        rocblas_datatype_f16_r, (rocblas_int)g.ldb, g.sb, &beta, C, rocblas_datatype_f32_r, (rocblas_int)g.ldc, g.sc,
+        C, rocblas_datatype_f32_r, (rocblas_int)g.ldc, g.sc, g.batch, rocblas_datatype_f32_r,
+        rocblas_gemm_algo_standard, 0, 0);
+    GMC_CHECK(rs == rocblas_status_success, "rocBLAS f16 gemm failed with status %d", (int)rs);
+    return 0;
+}
+
 template int gemm_big<double>(hipStream_t, const GemmShape&, double, const double*, const double*, double, double*);
 template int gemm_big<float>(hipStream_t, const GemmShape&, float, const float*, const float*, float, float*);
 

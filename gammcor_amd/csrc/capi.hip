@@ -197,6 +197,10 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             if (const char* e = getenv("GMC_OA_STEPS")) n_min = n_max = std::max(0, atoi(e));
             float oa_extend = OA_EXTEND;
             if (const char* e = getenv("GMC_OA_EXTEND")) oa_extend = (float)std::max(0.0, atof(e));
+            // GMC_OA_HALF: the correction from which the update runs in half
+            // precision (diagnostics; a large value keeps every update FP32)
+            float half_from = OA_HALF;
+            if (const char* e = getenv("GMC_OA_HALF")) half_from = (float)atof(e);
             const EigSolve64 resolve = [&](double* A, double* w, int nb) -> int {
                 EigPlan<double>* p64 = nullptr;
                 GMC_TRY(st.eig_plan_for<double>(nd, nb, &p64));
@@ -214,7 +218,7 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             GMC_TRY(st.erpaws.reserve(align_up(mb) + outb));
             bind_out(align_up(mb));
             GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_min,
-                                           n_max, oa_extend, n_max > n_min ? &resolve : nullptr));
+                                           n_max, oa_extend, n_max > n_min ? &resolve : nullptr, half_from));
             // rare and as large as an FP64 eigensolver's workspace: not kept
             // beyond the call that needed it
             if (st.resolve.ptr) {

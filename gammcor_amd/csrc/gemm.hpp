@@ -27,6 +27,10 @@ template <class T>
 int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
 void blas_release(void* handle);
 
+// C (FP32) = alpha * op(A) * op(B) + beta * C with A, B in IEEE half
+// precision and FP32 accumulation (rocBLAS gemm_ex, full products only).
+int gemm_f16(hipStream_t s, const GemmShape& g, float alpha, const void* A, const void* B, float beta, float* C);
+
 // Same, with one pointer per batch entry (device arrays of pointers).
 template <class T>
 int gemm_ptr(hipStream_t s, const GemmShape& g, T alpha, const T* const* A, const T* const* B, T beta,
