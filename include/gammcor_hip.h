@@ -26,8 +26,9 @@ extern "C" {
 #define GMC_FP64 0    /* plain double precision throughout                     */
 #define GMC_FP32 1    /* eigen solves in float, no refinement (diagnostic)     */
 #define GMC_MIXED 2   /* float eigen solves + Ogita-Aishima steps in double: three, */
-                      /* up to six while the corrections stay large; problems   */
-                      /* still not converged are solved again in double         */
+                      /* up to six while the corrections stay large (updates of */
+                      /* corrections >= 5e-3 from half inputs); problems still  */
+                      /* not converged are solved again in double               */
 #define GMC_MIXED2 3  /* float eigen solves + two Ogita-Aishima steps (faster,    */
                       /* misses 1 uEh at NBasis=240; diagnostic)                 */
 
