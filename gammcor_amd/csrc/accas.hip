@@ -866,12 +866,27 @@ __global__ void f64_to_f16_kernel(const double* a, __half* b, int64_t count, dou
         b[i] = __float2half((float)(a[i] * scale));   // via float: 2^-24 against half's 2^-11
 }
 
+// hi + 2^-11 lo split of a * scale in IEEE half precision: the lo part carries
+// the next 11 bits, scaled by 2^11 so it stays in half's normal range; for
+// |a * scale| <= 1 the pair represents it to about 3e-11 absolute.
+__global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int64_t count, double scale)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = (float)(a[i] * scale);
+        const __half h = __float2half(v);
+        hi[i] = h;
+        lo[i] = __float2half((v - __half2float(h)) * 2048.0f);
+    }
+}
+
 // The update X E is a correction of X: its rounding adds about u |X||E| to
 // an error the step leaves at about max|E|^2 (the step converges
 // quadratically).  A step with max|E| >= half_from forms it from IEEE half
 // inputs (u = 4.9e-4) with FP32 accumulation, which keeps u max|E| at or
-// below a tenth of max|E|^2; smaller corrections use FP32 (u = 6e-8).  The
-// residual terms (A X, X^T A X, X^T X) are FP64.
+// below a tenth of max|E|^2.  Smaller corrections split X and E / max|E|
+// into hi + 2^-11 lo half pairs and sum three half products (hi hi, hi lo,
+// lo hi; lo lo lies below 2^-22): about FP32 accuracy (u = 6e-8) on the half
+// MFMA rate.  The residual terms (A X, X^T A X, X^T X) are FP64.
 static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
                      float* E32, float* P32, float half_from)
@@ -899,7 +914,7 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     float emax;
     std::memcpy(&emax, &mxb, 4);
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
-    if (emax >= half_from && emax > 0.f) {
+    if (emax >= half_from) {
         // half inputs in the FP32 buffers' space, E scaled by 1 / emax
         __half* X16 = reinterpret_cast<__half*>(X32);
         __half* E16 = reinterpret_cast<__half*>(E32);
@@ -909,10 +924,22 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
         GMC_HIP(hipGetLastError());
         GMC_TRY(gemm_f16(st, gnn, emax, X16, E16, 0.0f, P32));
     } else {
-        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, X, X32, (int64_t)B * nn);
-        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), dim3(256), 0, st, AX, E32, (int64_t)B * nn);
+        // hi and lo halves of X in X32's space, of E / den in E32's.  den is
+        // at least 1e-4: the diagonal E_ii = r_ii / 2 (not in emax) is below
+        // 1e-6 from the FP32 start on, so E / den stays far inside half's range.
+        const float den = std::max(emax, 1e-4f);
+        __half* Xh = reinterpret_cast<__half*>(X32);
+        __half* Xl = Xh + (size_t)B * nn;
+        __half* Eh = reinterpret_cast<__half*>(E32);
+        __half* El = Eh + (size_t)B * nn;
+        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, X, Xh, Xl, (int64_t)B * nn, 1.0);
+        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, AX, Eh, El, (int64_t)B * nn,
+                           1.0 / (double)den);
         GMC_HIP(hipGetLastError());
-        GMC_TRY(gemm_big<float>(st, gnn, 1.0f, X32, E32, 0.0f, P32));
+        const float lo = den / 2048.0f;
+        GMC_TRY(gemm_f16(st, gnn, den, Xh, Eh, 0.0f, P32));
+        GMC_TRY(gemm_f16(st, gnn, lo, Xh, El, 1.0f, P32));
+        GMC_TRY(gemm_f16(st, gnn, lo, Xl, Eh, 1.0f, P32));
     }
     hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());

@@ -197,8 +197,9 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             if (const char* e = getenv("GMC_OA_STEPS")) n_min = n_max = std::max(0, atoi(e));
             float oa_extend = OA_EXTEND;
             if (const char* e = getenv("GMC_OA_EXTEND")) oa_extend = (float)std::max(0.0, atof(e));
-            // GMC_OA_HALF: the correction from which the update runs in half
-            // precision (diagnostics; a large value keeps every update FP32)
+            // GMC_OA_HALF: the correction from which the update runs from
+            // plain half inputs (diagnostics; below it the update sums three
+            // products of hi/lo half splits)
             float half_from = OA_HALF;
             if (const char* e = getenv("GMC_OA_HALF")) half_from = (float)atof(e);
             const EigSolve64 resolve = [&](double* A, double* w, int nb) -> int {
