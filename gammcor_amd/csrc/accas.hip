@@ -925,8 +925,9 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
         GMC_TRY(gemm_f16(st, gnn, emax, X16, E16, 0.0f, P32));
     } else {
         // hi and lo halves of X in X32's space, of E / den in E32's.  den is
-        // at least 1e-4: the diagonal E_ii = r_ii / 2 (not in emax) is below
-        // 1e-6 from the FP32 start on, so E / den stays far inside half's range.
+        // at least 1e-4: the diagonal E_ii = r_ii / 2 (not in emax) is a few
+        // 1e-6 at most from the FP32 start on (orthogonality ~3e-6), so E / den
+        // stays far inside half's range.
         const float den = std::max(emax, 1e-4f);
         __half* Xh = reinterpret_cast<__half*>(X32);
         __half* Xl = Xh + (size_t)B * nn;
