@@ -867,8 +867,9 @@ __global__ void f64_to_f16_kernel(const double* a, __half* b, int64_t count, dou
 }
 
 // hi + 2^-11 lo split of a * scale in IEEE half precision: the lo part carries
-// the next 11 bits, scaled by 2^11 so it stays in half's normal range; for
-// |a * scale| <= 1 the pair represents it to about 3e-11 absolute.
+// the next 11 bits, scaled by 2^11 so it stays in half's normal range; the
+// pair represents the value to 2^-22 relative (1.5e-11 absolute below half's
+// normal range).
 __global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int64_t count, double scale)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
@@ -885,8 +886,8 @@ __global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int
 // inputs (u = 4.9e-4) with FP32 accumulation, which keeps u max|E| at or
 // below a tenth of max|E|^2.  Smaller corrections split X and E / max|E|
 // into hi + 2^-11 lo half pairs and sum three half products (hi hi, hi lo,
-// lo hi; lo lo lies below 2^-22): about FP32 accuracy (u = 6e-8) on the half
-// MFMA rate.  The residual terms (A X, X^T A X, X^T X) are FP64.
+// lo hi; lo lo lies below 2^-22): close to FP32 accuracy (2^-22 against
+// 2^-24) on the half MFMA rate.  The residual terms (A X, X^T A X, X^T X) are FP64.
 static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
                      float* E32, float* P32, float half_from)
