@@ -334,6 +334,41 @@ def test_ac0_planted_mistakes_are_caught(monkeypatch):
     monkeypatch.setattr(O, "erpasymm0", e0)
 
 
+def test_ac0_alpha0_part_cancels(monkeypatch):
+    """In the alpha=0 eigenbasis X^T A+(0) X = Y^T A-(0) Y = diag(omega)/2
+    (ERPASYMM0's normalisation 2 Y^T X = 1 with X = omega^-1 A- Y), so the
+    first-order term is blind to A(0) inside the slope: A(1) in place of
+    A(1) - A(0) leaves AC0's ECorr unchanged.  With X off by a factor the
+    identity breaks."""
+    import accas_oracle as O
+    fx = gold("n16_cas22_ac0")
+    s = system_of(fx)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    sd = O.system_dict(s)
+    orig = O.ab_cas
+
+    def a1_plus_a0(*a, **k):
+        # alpha = 1 returns A(1) + A(0): the slope A(1) - A(0) becomes A(1)
+        ap, am, e, c = orig(*a, **k)
+        if a[-1] == 1.0:
+            bp, bm, _, _ = orig(*a[:-1], 0.0, **k)
+            return ap + bp, am + bm, e, c
+        return ap, am, e, c
+
+    def shift(monkeypatch):
+        base = O.ac0cas(sd, tw)["ecorr"]
+        monkeypatch.setattr(O, "ab_cas", a1_plus_a0)
+        shifted = O.ac0cas(sd, tw)["ecorr"]
+        monkeypatch.setattr(O, "ab_cas", orig)
+        return abs(shifted - base) / max(1.0, abs(base))
+
+    assert shift(monkeypatch) < 1e-10
+    e0 = O.erpasymm0
+    monkeypatch.setattr(O, "erpasymm0", lambda ap, am: (lambda y, x, e: (y, 2.0 * x, e))(*e0(ap, am)))
+    assert shift(monkeypatch) > 1e-6
+    monkeypatch.setattr(O, "erpasymm0", e0)
+
+
 # ---------------------------------------------------------------- rdm2.dat
 def test_rdm2_dat_round_trip():
     from gammcor_amd import synth, host
