@@ -5,6 +5,7 @@ module raises, there is no fallback implementation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -66,7 +67,16 @@ def lib():
         l.gmc_last_error.argtypes = []
         l.gmc_last_error.restype = ctypes.c_char_p
         _lib = l
+        # release the library's device memory, rocBLAS handle, events and
+        # stream while the HIP runtime (and a profiler attached to it) is
+        # still up: Python's exit handlers run before the runtime's teardown
+        atexit.register(_finalize)
     return _lib
+
+
+def _finalize():
+    if _lib is not None:
+        _lib.gmc_finalize()
 
 
 def check(rc):

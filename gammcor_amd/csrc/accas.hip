@@ -15,6 +15,7 @@
 #include <hip/hip_fp16.h>
 
 #include "accas.hpp"
+#include "oa_policy.hpp"
 #include "gemm.hpp"
 #include "prof.hpp"
 
@@ -742,23 +743,27 @@ __global__ void oa_delta_kernel(const double* partial, int np, double* delta, in
 // A close pair keeps only the orthogonalising r_ij / 2 here and is recorded
 // (lower index, upper index, Jacobi cosine and sine of its 2x2 block) so that
 // oa_pair_rot_kernel resolves it exactly after the update.
+// At most OA_MAXPAIRS close pairs per problem are recorded (the first `cap`,
+// cap <= OA_MAXPAIRS); npair keeps counting beyond, and a problem with more
+// pairs than recorded is treated as not converged (oa_policy.hpp).
 constexpr int OA_MAXPAIRS = 4096;
 constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
 constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
 
 __device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                          unsigned* npair, int4* pairs, double2* rot, int b, int i, int j);
+                          unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j);
 
 __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
                                                    const double* delta, double* E, int n, unsigned* npair, int4* pairs,
-                                                   double2* rot, unsigned* emax)
+                                                   double2* rot, unsigned cap, unsigned* emax)
 {
     const int b = blockIdx.z;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     // largest off-diagonal correction of the block, one atomic per block
+    // (oa_e_one returns +inf for a NaN element, so fmaxf keeps it)
     __shared__ float wmax[4];
     float my = 0.f;
-    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, b, i, j);
+    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, cap, b, i, j);
     for (int o = 32; o > 0; o >>= 1) my = fmaxf(my, __shfl_xor(my, o, 64));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = my;
     __syncthreads();
@@ -768,9 +773,12 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double
     }
 }
 
-// One element of E; returns |E_ij| for i != j (0 on the diagonal).
+// One element of E; returns |E_ij| for i != j (0 on the diagonal), +inf
+// when it is not finite.  The separated-or-close test is symmetric in (i, j)
+// (the larger of |s_ij + l_j r_ij| and |s_ij + l_i r_ij|), so E_ij and E_ji
+// always take the same branch.
 __device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                          unsigned* npair, int4* pairs, double2* rot, int b, int i, int j)
+                          unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j)
 {
     const size_t base = (size_t)b * n * n;
     const int hi = i > j ? i : j, lo = i > j ? j : i;
@@ -779,8 +787,9 @@ __device__ float oa_e_one(const double* S, const double* G, const double* lam, c
     const double rij = (i == j ? 1.0 : 0.0) - G[at];
     const double li = lam[(size_t)b * n + i], lj = lam[(size_t)b * n + j];
     const double num = sij + lj * rij, den = lj - li;
+    const double nsym = fmax(fabs(num), fabs(sij + li * rij));
     double e;
-    if (i != j && fabs(den) > fmax(8.0 * fabs(num), delta[b])) {
+    if (i != j && fabs(den) > fmax(8.0 * nsym, delta[b])) {
         e = num / den;
     } else {
         e = 0.5 * rij;
@@ -793,7 +802,7 @@ __device__ float oa_e_one(const double* S, const double* G, const double* lam, c
                 const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                 const double cs = 1.0 / sqrt(1.0 + t * t);
                 const unsigned slot = atomicAdd(&npair[b], 1u);
-                if (slot < OA_MAXPAIRS) {
+                if (slot < cap) {
                     pairs[(size_t)b * OA_MAXPAIRS + slot] = make_int4(lo, hi, 0, 0);
                     rot[(size_t)b * OA_MAXPAIRS + slot] = make_double2(cs, t * cs);
                 }
@@ -801,16 +810,18 @@ __device__ float oa_e_one(const double* S, const double* G, const double* lam, c
         }
     }
     E[base + i + (size_t)j * n] = e;
-    return i == j ? 0.f : (float)fabs(e);
+    if (i == j) return 0.f;
+    const float r = (float)fabs(e);
+    return r <= 3.4028235e38f ? r : INFINITY;
 }
 
 // x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
 // close pair (in record order), one workgroup per problem.
 __global__ __launch_bounds__(256) void oa_pair_rot_kernel(double* X, int n, const unsigned* npair, const int4* pairs,
-                                                          const double2* rot)
+                                                          const double2* rot, unsigned cap)
 {
     const int b = blockIdx.x;
-    const unsigned np = min(npair[b], (unsigned)OA_MAXPAIRS);
+    const unsigned np = min(npair[b], cap);
     double* x = X + (size_t)b * n * n;
     for (unsigned k = 0; k < np; k++) {
         const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
@@ -826,11 +837,11 @@ __global__ __launch_bounds__(256) void oa_pair_rot_kernel(double* X, int n, cons
 
 // Eigenvalue estimates of the rotated pairs: a - t c, b + t c.
 __global__ void oa_pair_lam_kernel(double* lam, const double* S, const double* G, int n, const unsigned* npair,
-                                   const int4* pairs, const double2* rot)
+                                   const int4* pairs, const double2* rot, unsigned cap)
 {
     const int b = blockIdx.x;
     if (threadIdx.x != 0) return;
-    const unsigned np = min(npair[b], (unsigned)OA_MAXPAIRS);
+    const unsigned np = min(npair[b], cap);
     for (unsigned k = 0; k < np; k++) {
         const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
         const double2 cr = rot[(size_t)b * OA_MAXPAIRS + k];
@@ -850,6 +861,7 @@ struct OaPairs {
     int4* pairs;        // B x OA_MAXPAIRS
     double2* rot;       // B x OA_MAXPAIRS
     unsigned* emax;     // B x OA_EMAX_SLOTS: float bits of max |E_ij| (i != j) of the last step
+    unsigned cap = OA_MAXPAIRS;   // pairs recorded per problem (<= OA_MAXPAIRS)
 };
 
 // Xn = X + P, P the FP32 product X E of the update.
@@ -888,9 +900,11 @@ __global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int
 // into hi + 2^-11 lo half pairs and sum three half products (hi hi, hi lo,
 // lo hi; lo lo lies below 2^-22): close to FP32 accuracy (2^-22 against
 // 2^-24) on the half MFMA rate.  The residual terms (A X, X^T A X, X^T X) are FP64.
+// pe receives each problem's correction of the step (oa_effective: +inf for
+// a non-finite step or more close pairs than recorded).
 static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
-                     float* E32, float* P32, float half_from)
+                     float* E32, float* P32, float half_from, std::vector<float>& pe)
 {
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
@@ -903,17 +917,26 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
     GMC_HIP(hipMemsetAsync(op.emax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
-                       op.pairs, op.rot, op.emax);
+                       op.pairs, op.rot, op.cap, op.emax);
     GMC_HIP(hipGetLastError());
-    // the largest off-diagonal correction of the batch (non-negative floats
-    // order as their bits)
-    std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
+    // each problem's largest off-diagonal correction (non-negative floats
+    // order as their bits; +inf marks a non-finite element) and pair count
+    std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS), np(B);
     GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipMemcpyAsync(np.data(), op.npair, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipStreamSynchronize(st));
-    unsigned mxb = 0;
-    for (unsigned u : em) mxb = std::max(mxb, u);
-    float emax;
-    std::memcpy(&emax, &mxb, 4);
+    pe.assign(B, 0.f);
+    // the scale of the update: the largest finite correction of the batch (a
+    // problem at +inf is solved again or fails the call, its update is moot)
+    float emax = 0.f;
+    for (int b = 0; b < B; b++) {
+        unsigned mxb = 0;
+        for (int k = 0; k < OA_EMAX_SLOTS; k++) mxb = std::max(mxb, em[(size_t)b * OA_EMAX_SLOTS + k]);
+        float e;
+        std::memcpy(&e, &mxb, 4);
+        if (e <= 3.4028235e38f) emax = std::max(emax, e);
+        pe[b] = oa_effective(e, np[b], op.cap);
+    }
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
     if (emax >= half_from) {
         // half inputs in the FP32 buffers' space, E scaled by 1 / emax
@@ -946,8 +969,8 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
-    hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot);
-    hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot);
+    hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot, op.cap);
+    hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot, op.cap);
     GMC_HIP(hipGetLastError());
     return 0;
 }
@@ -1004,6 +1027,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     float* X32 = ar.take<float>(B * nn);
     float* P32 = ar.take<float>(B * nn);
     op.emax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
+    // GMC_OA_MAXPAIRS lowers the number of close pairs recorded per problem
+    // (diagnostics: forces the over-the-cap path)
+    if (const char* e = getenv("GMC_OA_MAXPAIRS")) op.cap = (unsigned)std::min(std::max(0, atoi(e)), OA_MAXPAIRS);
     double* wre = ar.take<double>(B * n);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
@@ -1026,48 +1052,27 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems, max |E| %.3e\n", what, it, tot, B, mx);
         return 0;
     };
-    // Largest correction of the last step, per problem
+    // Largest correction of the last step, per problem (oa_effective)
     std::vector<float> pe(B);
-    auto read_emax = [&]() -> int {
-        std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
-        GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        GMC_HIP(hipStreamSynchronize(st));
-        for (int b = 0; b < B; b++) {
-            unsigned mx = 0;   // non-negative floats order as their bits
-            for (int k = 0; k < OA_EMAX_SLOTS; k++) mx = std::max(mx, em[(size_t)b * OA_EMAX_SLOTS + k]);
-            std::memcpy(&pe[b], &mx, 4);
-        }
-        return 0;
-    };
-    // Refinement of X and lamv against the FP64 matrices A.  Steps 0 ..
-    // n_min-1 always run unless every correction drops below OA_TOL; after
-    // them, more run while a correction is at least oa_extend, up to n_max.
-    // With `resolve`, the problems still at or above oa_extend when it stops
-    // go to `left`; it stops early once all of them are at or above OA_GIVEUP.
+    // Refinement of X and lamv against the FP64 matrices A, step by step until
+    // oa_decide (oa_policy.hpp) stops it; with `resolve`, the problems it
+    // leaves unconverged go to `left`.  Without one, a problem whose
+    // corrections are not finite or whose close pairs overflowed the record
+    // fails the call.
     auto refine = [&](const char* what, const double* A, double* lamv, double* axbuf, std::vector<int>& left) -> int {
         left.clear();
         for (int it = 0; it < n_max; it++) {
             GMC_TRY(oa_refine(st, n, B, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32, P32,
-                              half_from));
+                              half_from, pe));
             GMC_TRY(report(what, it));
-            const bool last = it + 1 >= n_max;
-            if (last && !resolve) break;
-            GMC_TRY(read_emax());
-            const float mx = *std::max_element(pe.begin(), pe.end());
-            if (it + 1 < n_min) {
-                if (mx < OA_TOL) break;
-                continue;
-            }
-            if (mx < oa_extend) break;
-            bool giveup = resolve != nullptr;
-            for (float e : pe)
-                if (e >= oa_extend && e < OA_GIVEUP) giveup = false;
-            if (last || giveup) {
-                if (resolve)
-                    for (int b = 0; b < B; b++)
-                        if (pe[b] >= oa_extend) left.push_back(b);
-                break;
-            }
+            const OaDecision d = oa_decide(pe, it, n_min, n_max, OA_TOL, oa_extend, OA_GIVEUP, resolve != nullptr);
+            if (d.next == OaNext::Continue) continue;
+            if (d.next == OaNext::Resolve) left = d.left;
+            GMC_CHECK(d.next != OaNext::Fail,
+                      "ERPA mixed: refinement of %s left %d problem(s) (first %d) with non-finite corrections or more "
+                      "than %u close pairs after %d steps; GMC_MIXED solves such problems again in FP64", what,
+                      (int)d.left.size(), d.left.empty() ? -1 : d.left[0], op.cap, n_max);
+            break;
         }
         return 0;
     };

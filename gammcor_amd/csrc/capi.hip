@@ -14,6 +14,7 @@
 #include "accas.hpp"
 #include "ac0.hpp"
 #include "fofo.hpp"
+#include "prof.hpp"
 
 namespace gmc {
 
@@ -303,6 +304,7 @@ int gmc_finalize(void)
     State& st = state();
     if (!st.ready) return 0;
     GMC_HIP(hipStreamSynchronize(st.stream));
+    prof_release();
     st.release();
     GMC_HIP(hipStreamDestroy(st.stream));
     st.ready = false;

@@ -35,6 +35,20 @@ hipEvent_t take_event()
 
 bool prof_enabled() { return g_on; }
 
+void prof_release()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_on = false;
+    for (auto& r : g_recs) {
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : g_pool) (void)hipEventDestroy(e);
+    g_recs.clear();
+    g_pool.clear();
+}
+
 ProfScope::ProfScope(hipStream_t st, const char* name, double work) : s(st)
 {
     if (!g_on) return;

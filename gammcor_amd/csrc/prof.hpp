@@ -16,5 +16,7 @@ struct ProfScope {
 };
 
 bool prof_enabled();
+// Destroys every event of the registry (gmc_finalize).
+void prof_release();
 
 }  // namespace gmc

@@ -261,9 +261,14 @@ def unpack_sym(v, n):
     return m
 
 
-def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=True, **_):
+def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=True, ci_ref_weight=None, **_):
+    """One seeded system.  ci_ref_weight: |c|^2 put on the leading determinant
+    of the CI vector (None: CI_REF_WEIGHT; 0: a plain normalised Gaussian
+    vector over all determinants)."""
+    if ci_ref_weight is None:
+        ci_ref_weight = CI_REF_WEIGHT
     s_l, s_c, s_ci, s_h = seed * 10 + 0, seed * 10 + 1, seed * 10 + 2, seed * 10 + 3
-    gamma, d = cas_rdms(nact, nelact, s_ci)
+    gamma, d = cas_rdms(nact, nelact, s_ci, ref_weight=ci_ref_weight)
     w, v = np.linalg.eigh(gamma) if nact else (np.zeros(0), np.zeros((0, 0)))
     order = np.argsort(-w, kind="stable")
     w, v = w[order], v[:, order]
@@ -289,7 +294,7 @@ def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=
     sysm.meta = dict(nbasis=nbasis, ninact=ninact, nact=nact, nelact=nelact, rank=rank, sigma=sigma,
                      seed=seed, seeds=dict(chol=s_l, orbitals=s_c, ci=s_ci, hcore=s_h),
                      h_sigma=H_SIGMA, gap=GAP, widths=[W_INACT, W_ACT, W_VIRT],
-                     ci_ref_weight=CI_REF_WEIGHT, enuc=0.0)
+                     ci_ref_weight=ci_ref_weight, enuc=0.0)
     sysm.chol = chol
     if with_integrals:
         sysm.twoel_ao = ao_integrals_packed(chol)

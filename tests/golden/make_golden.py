@@ -12,6 +12,7 @@ stored: they are regenerated from the seeds recorded in each fixture.
 
     python tests/golden/make_golden.py small      # N <= 48 cases (seconds)
     python tests/golden/make_golden.py n160       # NBasis=160 config (minutes)
+    python tests/golden/make_golden.py n240       # NBasis=240 config (about an hour, 25 GB)
 """
 from __future__ import annotations
 
@@ -39,15 +40,21 @@ CASES = {
     "n24_cas66": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "grid", 4),
     "n48_cas44": (dict(synth.CONFIGS["n48_cas44"]), "grid", 8),
     "n48_cas44_accas": (dict(synth.CONFIGS["n48_cas44"]), "accas", 5),
+    # the NBasis=48 system with a plain normalised Gaussian CI vector (no
+    # weight put on the leading determinant)
+    "n48_cas44_plainci": (dict(synth.CONFIGS["n48_cas44"], ci_ref_weight=0.0), "grid", 8),
     "n12_cas44_ab": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "dumpab", 1),
     "n160_cas88": (dict(synth.CONFIGS["n160_cas88"]), "grid", 8),
+    # the NBasis=240 workload with a 4-point rule: the reference needs about
+    # 25 GB and an hour here, the GPU test compares every W(alpha) and ECorr
+    "n240_cas88": (dict(synth.CONFIGS["n240_cas88"]), "grid", 4),
     "n12_cas44_ac0": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "ac0", 1),
     "n16_cas22_ac0": (dict(nbasis=16, ninact=3, nact=2, nelact=2, rank=30, sigma=0.1, seed=1), "ac0", 1),
     "n20_cas44_noinact_ac0": (dict(nbasis=20, ninact=0, nact=4, nelact=4, rank=50, sigma=0.1, seed=2), "ac0", 1),
     "n24_cas66_ac0": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "ac0", 1),
     "n48_cas44_ac0": (dict(synth.CONFIGS["n48_cas44"]), "ac0", 1),
 }
-SMALL = [k for k in CASES if not k.startswith("n160")]
+SMALL = [k for k in CASES if not k.startswith(("n160", "n240"))]
 
 
 def parse(out):

@@ -21,14 +21,16 @@ def _device_system(fx, cuda):
     import torch
     from gammcor_amd import host, synth
     m = fx["system"]
-    s = synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"])
+    s = synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"],
+                          ci_ref_weight=m.get("ci_ref_weight"))
     tno = torch.from_numpy(s.twoel_ao).to(cuda)
     ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(cuda)
     host.TwoNO1(tno, ure, s.nbasis)
     return s, host.DeviceSystem(tno, s.occ, s.xone, s.gamma_act, s.nbasis, s.ninact, s.nact, s.nele)
 
 
-@pytest.mark.parametrize("name", ["n12_cas44", "n16_cas22", "n20_cas44_noinact", "n24_cas66", "n48_cas44"])
+@pytest.mark.parametrize("name", ["n12_cas44", "n16_cas22", "n20_cas44_noinact", "n24_cas66", "n48_cas44",
+                                  "n48_cas44_plainci"])
 def test_fp64_matches_reference(cuda, name):
     from gammcor_amd import host
     fx = gold(name)
@@ -59,7 +61,7 @@ def test_fp64_matches_reference_accas_5_point(cuda):
     assert abs(r["ecorr"] - fx["accas_line"][1]) < 1e-8
 
 
-@pytest.mark.parametrize("name", ["n24_cas66", "n48_cas44"])
+@pytest.mark.parametrize("name", ["n24_cas66", "n48_cas44", "n48_cas44_plainci"])
 def test_mixed_matches_reference(cuda, name):
     from gammcor_amd import host
     fx = gold(name)

@@ -54,3 +54,64 @@ def test_unconverged_problems_are_solved_again_in_fp64(cuda, monkeypatch, capfd)
     assert err.count("[gmc] OA M step") == 6
     assert abs(f["ecorr"] - ref["ecorr"]) < 1e-10
     assert np.max(np.abs(f["eig"].cpu().numpy() - ref["eig"].cpu().numpy())) < 1e-10
+
+
+def _n48_system():
+    from gammcor_amd import host, synth
+    import accas_oracle as O
+    s = synth.make_system(48, 6, 4, 4, 144, 0.1, 0)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    a, w = host.GauLeg(0.0, 1.0, 8)
+    return host.DeviceSystem(tw, s.occ, s.xone, s.gamma_act, 48, 6, 4, s.nele), a, w
+
+
+def test_half_precision_updates_on_every_step(cuda, monkeypatch, capfd):
+    """GMC_OA_HALF at 1e-12 forms every refinement update from plain half
+    inputs (the branch larger systems take for their first corrections):
+    the refinement still meets 1 uEh against the FP64 path."""
+    from gammcor_amd import host
+    d, a, w = _n48_system()
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64)
+    monkeypatch.setenv("GMC_OA_HALF", "1e-12")
+    monkeypatch.setenv("GMC_OA_STATS", "1")
+    capfd.readouterr()
+    m = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+    err = capfd.readouterr().err
+    print(err, flush=True)
+    assert abs(m["ecorr"] - ref["ecorr"]) < 1e-6
+
+
+def test_close_pairs_over_the_cap_go_to_fp64(cuda, monkeypatch, capfd):
+    """With no close pair recorded (GMC_OA_MAXPAIRS=0) every problem that
+    finds one cannot be taken as converged: it is solved again in FP64, and
+    the mixed result stays within 1 uEh of the FP64 path."""
+    from gammcor_amd import host
+    d, a, w = _n48_system()
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64)
+    monkeypatch.setenv("GMC_OA_MAXPAIRS", "0")
+    monkeypatch.setenv("GMC_OA_STATS", "1")
+    capfd.readouterr()
+    m = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+    err = capfd.readouterr().err
+    print(err, flush=True)
+    assert abs(m["ecorr"] - ref["ecorr"]) < 1e-6
+    # a step that found close pairs leaves its problems unconverged
+    import re
+    steps = [(what, int(npair)) for what, npair in re.findall(r"OA (A\+|M) step \d+: (\d+) close pairs", err)]
+    for what in ("A+", "M"):
+        if any(npair > 0 for w_, npair in steps if w_ == what):
+            assert f"[gmc] OA {what}:" in err and "solved again in FP64" in err
+
+
+def test_fixed_step_count_fails_loudly_over_the_cap(cuda, monkeypatch):
+    """A fixed step count (GMC_MIXED2) has no FP64 re-solve: a problem left
+    with unrecorded close pairs fails the call instead of returning a value
+    its refinement did not converge to."""
+    from gammcor_amd import host, _lib
+    d, a, w = _n48_system()
+    monkeypatch.setenv("GMC_OA_MAXPAIRS", "0")
+    monkeypatch.setenv("GMC_OA_STATS", "1")
+    try:
+        host.ACECORR(d, a, w, prec=host.GMC_MIXED2)
+    except _lib.GmcError as e:
+        assert "close pairs" in str(e)

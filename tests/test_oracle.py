@@ -19,10 +19,11 @@ def gold(name):
 def system_of(fx):
     from gammcor_amd import synth
     m = fx["system"]
-    return synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"])
+    return synth.make_system(m["nbasis"], m["ninact"], m["nact"], m["nelact"], m["rank"], m["sigma"], m["seed"],
+                             ci_ref_weight=m.get("ci_ref_weight"))
 
 
-SMALL_GRID = ["n12_cas44", "n16_cas22", "n20_cas44_noinact", "n24_cas66", "n48_cas44"]
+SMALL_GRID = ["n12_cas44", "n16_cas22", "n20_cas44_noinact", "n24_cas66", "n48_cas44", "n48_cas44_plainci"]
 
 
 # ---------------------------------------------------------------- GauLeg
