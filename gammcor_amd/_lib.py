@@ -28,6 +28,13 @@ SIGNATURES = {
     "gmc_acecorr_fofo_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_syevd_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
+    "gmc_syevd_noalloc_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
+    "gmc_read_aotwoint_molpro": [ctypes.c_char_p, _c_int, _vp],
+    "gmc_read_rdm2_molpro": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp],
+    "gmc_read_rdm1_molpro": [ctypes.c_char_p, _c_int, _c_int, _c_int, _vp],
+    "gmc_read_mo_molpro": [ctypes.c_char_p, ctypes.c_char_p, _c_int, _vp],
+    "gmc_read_oneint_molpro": [ctypes.c_char_p, ctypes.c_char_p, _c_int, _vp],
+    "gmc_read_dpqrs": [ctypes.c_char_p, _c_int, _c_int, _vp],
     "gmc_prof_enable": [_c_int],
     "gmc_prof_reset": [],
     "gmc_prof_get": [ctypes.c_char_p, _vp, _vp, _vp],

@@ -1200,6 +1200,7 @@ __global__ void copy_kernel(const T* src, T* dst, int64_t count)
 template <class T>
 EigPlan<T>::~EigPlan()
 {
+    if (!owns_tables) return;
     for (auto& l : levels)
         if (l.d_desc) (void)hipFree(l.d_desc);
     if (d_split) (void)hipFree(d_split);
@@ -1239,8 +1240,11 @@ static int eig_layout(EigPlan<T>* p, char* base)
     return 0;
 }
 
+// The plan's host part: G, the divide & conquer levels with their merge
+// descriptors (h_desc, one vector per level) and split flags (h_split).
 template <class T>
-int eig_plan(int n, int batch, EigPlan<T>* p)
+static int eig_plan_host(int n, int batch, EigPlan<T>* p, std::vector<std::vector<int>>& h_desc,
+                         std::vector<unsigned char>& h_split)
 {
     GMC_CHECK(n >= 1 && batch >= 1, "eig_plan: bad sizes n=%d batch=%d", n, batch);
     p->n = n;
@@ -1291,11 +1295,9 @@ int eig_plan(int n, int batch, EigPlan<T>* p)
         L.nmerge = (int)lv.size();
         L.tiles = off;
         L.maxm = maxm;
-        GMC_HIP(hipMalloc(&L.d_desc, desc.size() * sizeof(int)));
-        GMC_HIP(hipMemcpy(L.d_desc, desc.data(), desc.size() * sizeof(int), hipMemcpyHostToDevice));
+        h_desc.push_back(std::move(desc));
     }
-    GMC_HIP(hipMalloc(&p->d_split, split.size()));
-    GMC_HIP(hipMemcpy(p->d_split, split.data(), split.size(), hipMemcpyHostToDevice));
+    h_split = std::move(split);
 
     // back-transformation block: the rank-nbb updates of Z read and write Z
     // once per block, so the block is as wide as the T factor in LDS allows
@@ -1303,6 +1305,56 @@ int eig_plan(int n, int batch, EigPlan<T>* p)
     GMC_TRY(eig_layout(p, nullptr));
     return 0;
 }
+
+template <class T>
+int eig_plan(int n, int batch, EigPlan<T>* p)
+{
+    std::vector<std::vector<int>> h_desc;
+    std::vector<unsigned char> h_split;
+    GMC_TRY(eig_plan_host(n, batch, p, h_desc, h_split));
+    p->owns_tables = true;
+    for (size_t h = 0; h < h_desc.size(); h++) {
+        DcLevel& L = p->levels[h];
+        GMC_HIP(hipMalloc(&L.d_desc, h_desc[h].size() * sizeof(int)));
+        GMC_HIP(hipMemcpy(L.d_desc, h_desc[h].data(), h_desc[h].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    GMC_HIP(hipMalloc(&p->d_split, h_split.size()));
+    GMC_HIP(hipMemcpy(p->d_split, h_split.data(), h_split.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <class T>
+int eig_plan_at(int n, int batch, EigPlan<T>* p, void* mem, size_t mem_bytes, size_t* used, hipStream_t s)
+{
+    std::vector<std::vector<int>> h_desc;
+    std::vector<unsigned char> h_split;
+    GMC_TRY(eig_plan_host(n, batch, p, h_desc, h_split));
+    p->owns_tables = false;
+    // the tables, then the workspace (bound by the caller at mem + *used)
+    size_t need = 0;
+    for (auto& d : h_desc) need = align_up(need) + d.size() * sizeof(int);
+    need = align_up(align_up(need) + h_split.size()) + p->ws_bytes;
+    GMC_CHECK(need <= mem_bytes, "syevd: n=%d, batch=%d needs %zu bytes of device scratch, more than the %zu "
+              "available", n, batch, need, mem_bytes);
+    char* base = static_cast<char*>(mem);
+    size_t off = 0;
+    for (size_t h = 0; h < h_desc.size(); h++) {
+        const size_t bytes = h_desc[h].size() * sizeof(int);
+        off = align_up(off);
+        p->levels[h].d_desc = reinterpret_cast<int*>(base + off);
+        GMC_HIP(hipMemcpyAsync(base + off, h_desc[h].data(), bytes, hipMemcpyHostToDevice, s));
+        off += bytes;
+    }
+    off = align_up(off);
+    p->d_split = reinterpret_cast<unsigned char*>(base + off);
+    GMC_HIP(hipMemcpyAsync(base + off, h_split.data(), h_split.size(), hipMemcpyHostToDevice, s));
+    off += h_split.size();
+    // the host vectors go out of scope: the copies must have left them
+    GMC_HIP(hipStreamSynchronize(s));
+    *used = align_up(off);
+    return 0;
+}
+
 
 template <class T>
 int eig_bind(EigPlan<T>* p, void* ws)
@@ -1446,13 +1498,16 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             GMC_HIP(hipGetLastError());
             GemmShape g2{kb, n, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
                          (int64_t)nn, B, 0};
-            GMC_TRY(gemm_big<T>(s, g2, T(1), V, Zs, T(0), p.QK));
+            // own_gemm: the library's MFMA tile (no library workspace)
+            if (p.own_gemm) GMC_TRY(gemm<T>(s, g2, T(1), V, Zs, T(0), p.QK));
+            else GMC_TRY(gemm_big<T>(s, g2, T(1), V, Zs, T(0), p.QK));
             GemmShape g3{kb, n, kb, false, false, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb * nbb,
                          (int64_t)nn, (int64_t)nn, B, 0};
             GMC_TRY(gemm<T>(s, g3, T(1), Tm, p.QK, T(0), p.QN));
             GemmShape g4{m, n, kb, false, false, (int64_t)n, (int64_t)nbb, (int64_t)n, (int64_t)nn, (int64_t)nn,
                          (int64_t)nn, B, 0};
-            GMC_TRY(gemm_big<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
+            if (p.own_gemm) GMC_TRY(gemm<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
+            else GMC_TRY(gemm_big<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
         }
     }
     hipLaunchKernelGGL(copy_kernel<T>, dim3(grid1((int64_t)B * nn)), dim3(256), 0, s, p.Q, A, (int64_t)B * nn);
@@ -1474,6 +1529,8 @@ template struct EigPlan<double>;
 template struct EigPlan<float>;
 template int eig_plan<double>(int, int, EigPlan<double>*);
 template int eig_plan<float>(int, int, EigPlan<float>*);
+template int eig_plan_at<double>(int, int, EigPlan<double>*, void*, size_t, size_t*, hipStream_t);
+template int eig_plan_at<float>(int, int, EigPlan<float>*, void*, size_t, size_t*, hipStream_t);
 template int eig_bind<double>(EigPlan<double>*, void*);
 template int eig_bind<float>(EigPlan<float>*, void*);
 template int syevd_batched<double>(hipStream_t, EigPlan<double>&, double*, double*);

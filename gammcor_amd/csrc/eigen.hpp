@@ -40,6 +40,8 @@ struct EigPlan {
     unsigned* bar = nullptr;
     int* err = nullptr;
     bool bound = false;
+    bool owns_tables = true;   // d_desc / d_split allocated by eig_plan (else placed by eig_plan_at)
+    bool own_gemm = false;     // back-transformation products on the library's MFMA tile only
     EigPlan() = default;
     EigPlan(const EigPlan&) = delete;
     EigPlan& operator=(const EigPlan&) = delete;
@@ -48,6 +50,14 @@ struct EigPlan {
 
 template <class T>
 int eig_plan(int n, int batch, EigPlan<T>* p);
+
+// The same plan with its divide & conquer tables placed at the start of the
+// caller's device memory `mem` and copied there on stream s (no device
+// allocation).  *used receives the bytes the tables take; mem_bytes must
+// also hold the workspace after them (the caller binds it at mem + *used),
+// else the call fails before copying anything.
+template <class T>
+int eig_plan_at(int n, int batch, EigPlan<T>* p, void* mem, size_t mem_bytes, size_t* used, hipStream_t s);
 
 // Size the caller's scratch for eig_plan: p->ws_bytes after eig_plan.
 template <class T>

@@ -296,3 +296,55 @@ def ACECORR_host(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2ac
               ig.ctypes.data, xo.ctypes.data, _ptr(u), rd.ctypes.data, len(indn), indn.ctypes.data, na,
               al.ctypes.data, wt.ctypes.data, ec.ctypes.data, wa.ctypes.data, es.ctypes.data, nneg.ctypes.data)
     return dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
+
+
+# ---------------------------------------------------------------- input files
+def _buf(n):
+    return np.zeros(n, dtype=np.float64)
+
+
+def read_aotwoint_molpro(path, nbasis):
+    """AO integrals of a Molpro AOTWOINT.mol (readtwoint, sorter.f90:133-313)
+    as the NAddr3-packed array TwoNO1 takes."""
+    t = nbasis * (nbasis + 1) // 2
+    out = _buf(t * (t + 1) // 2)
+    _lib.call("gmc_read_aotwoint_molpro", str(path).encode(), nbasis, out.ctypes.data)
+    return out
+
+
+def read_rdm2_molpro(path, nact, nost=1, nosym=0):
+    """RDM2Act (NAddrRDM packing) from a '2RDM' record (types.f90:622-731)."""
+    out = _buf(nact * nact * (nact * nact + 1) // 2)
+    _lib.call("gmc_read_rdm2_molpro", str(path).encode(), nact, nost, nosym, out.ctypes.data)
+    return out
+
+
+def read_rdm1_molpro(path, nact, nost=1, nosym=0):
+    """Halved active 1-RDM, column-packed upper triangle (types.f90:793-866)."""
+    out = _buf(nact * (nact + 1) // 2)
+    _lib.call("gmc_read_rdm1_molpro", str(path).encode(), nact, nost, nosym, out.ctypes.data)
+    return out
+
+
+def read_mo_molpro(path, nbasis, label="CASORB"):
+    """Orbital coefficients (column-major nbasis x nbasis) under `label` of
+    MOLPRO.MOPUN (types.f90:1078-1130)."""
+    out = _buf(nbasis * nbasis)
+    _lib.call("gmc_read_mo_molpro", str(path).encode(), label.encode(), nbasis, out.ctypes.data)
+    return out.reshape(nbasis, nbasis, order="F")
+
+
+def read_oneint_molpro(path, nbasis, label="ONEHAMIL"):
+    """Packed lower triangle of the one-electron integrals under `label` of
+    AOONEINT.mol (types.f90:1221-1283, expanded over all irreps)."""
+    out = _buf(nbasis * (nbasis + 1) // 2)
+    _lib.call("gmc_read_oneint_molpro", str(path).encode(), label.encode(), nbasis, out.ctypes.data)
+    return out
+
+
+def read_dpqrs(path, nbasis, header=False):
+    """DPQRS.bin (initia.f:440-466): the NAddr3-packed integrals."""
+    t = nbasis * (nbasis + 1) // 2
+    out = _buf(t * (t + 1) // 2)
+    _lib.call("gmc_read_dpqrs", str(path).encode(), nbasis, 1 if header else 0, out.ctypes.data)
+    return out

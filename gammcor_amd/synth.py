@@ -194,6 +194,25 @@ def cholesky_factors(nbasis, rank, sigma, seed):
     return rng.standard_normal((rank, t)) * sigma
 
 
+def cholesky_factors_sym(nt, rank, sigma, seed):
+    """Packed factors of integrals with point-group symmetry: the AOs come
+    in irrep blocks of sizes nt (irrep product = XOR of the 0-based labels,
+    len(nt) a power of two); factor P belongs to irrep P mod len(nt) and is
+    N(0, sigma^2) on the pairs (mu, nu) whose product is that irrep, zero
+    elsewhere, so (mu nu|la si) vanishes unless the product of all four
+    irreps is totally symmetric."""
+    nirr = len(nt)
+    assert nirr in (1, 2, 4, 8)
+    lab = np.repeat(np.arange(nirr), nt)
+    n = len(lab)
+    il, jl = np.tril_indices(n)
+    gam = lab[il] ^ lab[jl]                     # pair order i >= j, row-major == packed order
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((rank, len(il))) * sigma
+    f *= (gam[None, :] == (np.arange(rank) % nirr)[:, None])
+    return f
+
+
 def ao_integrals_packed(chol):
     """(ij|kl) = sum_P L[P,ij] L[P,kl] for ij >= kl, packed by NAddr3."""
     t = chol.shape[1]

@@ -187,6 +187,51 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
  * Replaces the DSYEVD behind Diag8 (misc.f:541-616) without its transpose. */
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
 
+/* ---- readers of the reference's input files (host code; no GPU) --------
+ * They return the arrays the entry points above take. */
+
+/* readtwoint with aosource 2 (Molpro's AOTWOINT.mol), sorter.f90:133-313:
+ * the symmetry-blocked AO integrals (all irreps in one file, AOs ordered by
+ * irrep) into the NAddr3-packed twono (NInte2 = T(T+1)/2, T = N(N+1)/2)
+ * that gmc_twono1 takes, instead of the reference's AOTWOSORT records. */
+int gmc_read_aotwoint_molpro(const char* path, int nbasis, double* twono);
+
+/* read_2rdm_molpro, types.f90:622-731: the active 2-RDM of state
+ * nost.nosym (nosym = 0: the file's first state) from the '2RDM' record
+ * into rdm2act (nact^2 (nact^2+1)/2 values, NAddrRDM packing, halved), the
+ * rdm2act argument of the entry points above. */
+int gmc_read_rdm2_molpro(const char* path, int nact, int nost, int nosym, double* rdm2act);
+
+/* read_1rdm_molpro, types.f90:793-866: the active 1-RDM of state
+ * nost.nosym, halved, column-packed upper triangle (nact (nact+1)/2). */
+int gmc_read_rdm1_molpro(const char* path, int nact, int nost, int nosym, double* onerdm);
+
+/* read_mo_molpro, types.f90:1078-1130: the orbitals under `label` (e.g.
+ * "CASORB") of MOLPRO.MOPUN, cmo column-major nbasis x nbasis. */
+int gmc_read_mo_molpro(const char* path, const char* label, int nbasis, double* cmo);
+
+/* readoneint_molpro with expand, types.f90:1221-1283: the one-electron
+ * integrals under `label` (e.g. "ONEHAMIL") of AOONEINT.mol as the packed
+ * lower triangle over all orbitals (N(N+1)/2). */
+int gmc_read_oneint_molpro(const char* path, const char* label, int nbasis, double* mone);
+
+/* ReadDMRG's DPQRS.bin, initia.f:440-466: NInte2 packed integrals as a raw
+ * stream (after three 4-byte integers when header != 0). */
+int gmc_read_dpqrs(const char* path, int nbasis, int header, double* twono);
+
+/* Allocation-free form of gmc_syevd_dev (same arguments and results).  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The workspace and the divide & conquer tables live in
+ * one static device array of GMC_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; a call makes no hipMalloc/hipFree and no allocating
+ * library call.  A problem set that needs more scratch (about
+ * 4 * batch * n^2 * sizeof(element) bytes, e.g. FP64 n = 2 800 at batch 1)
+ * fails with a message.  Calls that overlap on one GPU share the array and
+ * race.  Runs on the given stream and synchronises it before returning. */
+#define GMC_NOALLOC_SCRATCH_BYTES (256u * 1024u * 1024u)
+int gmc_syevd_noalloc_dev(int prec, int n, int batch, void* A, void* w, void* stream);
+
 /* Kernel timing (no counterpart in the reference; used by the benchmark):
  * while enabled, instrumented launches record HIP events on their stream.
  * gmc_prof_get sums, over the launches of one kernel name, the event time

@@ -17,7 +17,14 @@ C               as grid does;
 C       ac0   : AC0CAS (acfd.f:1902), the linearised AC correction, printing
 C               its ECorr and the CASSCF energy;
 C       ac0fofo: its FOFO version AC0CAS_FOFO (abfofo.f90:1485) on the FFOO
-C               and FOFO record files written as fofo does.
+C               and FOFO record files written as fofo does;
+C       formats: the reference's own readers of its input files on files
+C               written by gammcor_amd.formats (no transform, no solve):
+C               readtwoint with the Molpro source (sorter.f90:31) writes
+C               AOTWOSORT; read_2rdm_molpro, read_1rdm_molpro,
+C               read_mo_molpro and readoneint_molpro (types.f90) fill
+C               arrays written to ref_rdm2.bin, ref_rdm1.bin, ref_cmo.bin
+C               and ref_oneint.bin.
 C     Pair selection for grid/dumpab mirrors accas.f:30-99 with
 C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
       Program RefDrv
@@ -48,6 +55,11 @@ C
       NAl=Hdr(5)
       NInte1=NBasis*(NBasis+1)/2
       NInte2=NInte1*(NInte1+1)/2
+      If(Mode(1:7).Eq.'formats') Then
+      Close(11)
+      Call RefFormats(NBasis,NAct)
+      Stop
+      EndIf
       Allocate(Occ(NBasis),XOne(NInte1),UAOMO(NBasis,NBasis),
      $ URe(NBasis,NBasis),TwoNO(NInte2),AG(NAl),WG(NAl),
      $ UNOAO(NBasis,NBasis))
@@ -253,4 +265,41 @@ C
       EndDo
       Write(6,'(A,ES25.17)') 'ECORR ',ECorr
 C
+      End
+
+      Subroutine RefFormats(NBasis,NAct)
+C     The reference's readers on the files of gammcor_amd.formats; their
+C     results written as raw doubles for the test to compare.
+      use types
+      use sorter
+      Implicit None
+      Integer NBasis,NAct,IWarn,NTr
+      Real*8, Allocatable :: TwoRDM(:),OneRDM(:),CMO(:,:),XMone(:)
+      NTr=NBasis*(NBasis+1)/2
+      Allocate(TwoRDM(NAct**2*(NAct**2+1)/2),OneRDM(NTr),
+     $ CMO(NBasis,NBasis),XMone(NTr))
+      Call readtwoint(NBasis,2,'AOTWOINT.mol','AOTWOSORT')
+      IWarn=0
+      Call read_2rdm_molpro(TwoRDM,1,0,'RDM.mol',IWarn,NAct)
+      Call read_1rdm_molpro(OneRDM,1,0,'RDM.mol',IWarn,NBasis)
+      Call read_mo_molpro(CMO,'MOLPRO.MOPUN','CASORB  ',NBasis)
+      Call readoneint_molpro(XMone,'AOONEINT.mol','ONEHAMIL',.True.,
+     $ NTr)
+      Open(20,File='ref_rdm2.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='REPLACE')
+      Write(20) TwoRDM
+      Close(20)
+      Open(20,File='ref_rdm1.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='REPLACE')
+      Write(20) OneRDM
+      Close(20)
+      Open(20,File='ref_cmo.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='REPLACE')
+      Write(20) CMO
+      Close(20)
+      Open(20,File='ref_oneint.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='REPLACE')
+      Write(20) XMone
+      Close(20)
+      Write(6,'(A,I4)') 'FORMATS_OK ',IWarn
       End
