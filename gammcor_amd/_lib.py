@@ -23,6 +23,8 @@ SIGNATURES = {
     "gmc_twono1_dev": [_vp, _vp, _c_int, _c_i64, _vp],
     "gmc_ac0cas": [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int],
     "gmc_ac0cas_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp],
+    "gmc_ac0cas_fofo_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp],
+    "gmc_ac0cas_fofo": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int],
     "gmc_acecorr_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp,
                         _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_acecorr_fofo_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,

@@ -9,6 +9,11 @@
 // Both A(0) within a block and A' are the triangle at or below the diagonal
 // in that block order, mirrored (AB0ELEMENT is called for IRow >= ICol,
 // acfd.f:2168, 2244, 2323; AB1_CAS copies its lower triangle, 4200-4216).
+// Its FOFO form AC0CAS_FOFO (abfofo.f90:1485-1750) builds the same blocks
+// (create_AC0_blocks, 1374-1483) and the same first-order term, but from
+// the symmetrised whole of each alpha=0 block (ERPASYMM0 symmetrises) and of
+// A' (sq_symmetrize): the `symm` mode.  A(0), A(1) are not exactly
+// symmetric for a non-stationary reference, so the two differ.
 //
 // Here the pairs of the ERPA blocks are gathered into one n1 x n1 problem
 // (cross-block entries zero) and solved by the FP64 ERPA route; its Y and
@@ -32,28 +37,38 @@ namespace gmc {
 
 namespace {
 
-// out(i, j) = A(rows[i], rows[j]) taken at or below the diagonal in block
-// order (pos), zero across blocks (blk); i, j < m.
+// out(i, j) = A(rows[i], rows[j]) within a block (blk), zero across blocks;
+// i, j < m.  AC0CAS: the entry at or below the diagonal in block order
+// (pos), mirrored; AC0CAS_FOFO (symm): the entry as it is (the ERPA solve
+// symmetrises the block, as ERPASYMM0 does, interpa.f:1525-1533).
 __global__ void ac0_gather_kernel(const double* __restrict__ A, int n, const int* __restrict__ rows,
-                                  const int* __restrict__ blk, const int* __restrict__ pos, int m,
+                                  const int* __restrict__ blk, const int* __restrict__ pos, int m, int symm,
                                   double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= m) return;
     const int I = rows[i], J = rows[j];
     double v = 0.0;
-    if (blk[I] == blk[J]) v = pos[I] >= pos[J] ? A[I + (size_t)J * n] : A[J + (size_t)I * n];
+    if (blk[I] == blk[J])
+        v = (symm || pos[I] >= pos[J]) ? A[I + (size_t)J * n] : A[J + (size_t)I * n];
     out[i + (size_t)j * m] = v;
 }
 
-// A' = A(1) - A(0) over all pairs, the block-order lower triangle mirrored.
+// A' = A(1) - A(0) over all pairs: AC0CAS the block-order lower triangle
+// mirrored (AB1_CAS, acfd.f:4200-4216); AC0CAS_FOFO (symm) the symmetrised
+// whole (sq_symmetrize, abfofo.f90:268-272).
 __global__ void ac0_slope_kernel(const double* __restrict__ a0, const double* __restrict__ a1, int n,
-                                 const int* __restrict__ pos, double* __restrict__ out)
+                                 const int* __restrict__ pos, int symm, double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
-    const size_t at = pos[i] >= pos[j] ? i + (size_t)j * n : j + (size_t)i * n;
-    out[i + (size_t)j * n] = a1[at] - a0[at];
+    const size_t ij = i + (size_t)j * n, ji = j + (size_t)i * n;
+    if (symm) {
+        out[ij] = 0.5 * ((a1[ij] - a0[ij]) + (a1[ji] - a0[ji]));
+    } else {
+        const size_t at = pos[i] >= pos[j] ? ij : ji;
+        out[ij] = a1[at] - a0[at];
+    }
 }
 
 // Column k of the ERPA solution: X = omega^-1 (A- Y) for omega^2 > 1e-6, else
@@ -191,7 +206,8 @@ size_t ac0_bytes(int n, int n1)
 }
 
 int ac0_correction(hipStream_t s, const AbPair& ab, const double* Gw, int n, const Ac0Blocks& b, ErpaPlan<double>* pl,
-                   void* erpa_scratch, const ErpaOut& eout, void* scratch, size_t scratch_bytes, double* ecorr)
+                   void* erpa_scratch, const ErpaOut& eout, void* scratch, size_t scratch_bytes, double* ecorr,
+                   bool symm)
 {
     const int n1 = (int)b.rows1.size(), nvi = (int)b.vi.size();
     GMC_CHECK(n1 + nvi == n, "AC0CAS: block layout of %d + %d pairs for %d", n1, nvi, n);
@@ -223,14 +239,14 @@ int ac0_correction(hipStream_t s, const AbPair& ab, const double* Gw, int n, con
     GMC_HIP(hipMemcpyAsync(d_inv, b.inv1.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     const dim3 blk(256), gn((n + 255) / 256, n);
     // A'+- over all pairs
-    hipLaunchKernelGGL(ac0_slope_kernel, gn, blk, 0, s, ab.p0, ab.p1, n, d_pos, Dp);
-    hipLaunchKernelGGL(ac0_slope_kernel, gn, blk, 0, s, ab.m0, ab.m1, n, d_pos, Dm);
+    hipLaunchKernelGGL(ac0_slope_kernel, gn, blk, 0, s, ab.p0, ab.p1, n, d_pos, (int)symm, Dp);
+    hipLaunchKernelGGL(ac0_slope_kernel, gn, blk, 0, s, ab.m0, ab.m1, n, d_pos, (int)symm, Dm);
     GMC_HIP(hipGetLastError());
     if (n1) {
         // the ERPA blocks as one n1 x n1 problem at alpha = 0
         const dim3 g1((n1 + 255) / 256, n1);
-        hipLaunchKernelGGL(ac0_gather_kernel, g1, blk, 0, s, ab.p0, n, d_rows1, d_blk, d_pos, n1, Bp);
-        hipLaunchKernelGGL(ac0_gather_kernel, g1, blk, 0, s, ab.m0, n, d_rows1, d_blk, d_pos, n1, Bm);
+        hipLaunchKernelGGL(ac0_gather_kernel, g1, blk, 0, s, ab.p0, n, d_rows1, d_blk, d_pos, n1, (int)symm, Bp);
+        hipLaunchKernelGGL(ac0_gather_kernel, g1, blk, 0, s, ab.m0, n, d_rows1, d_blk, d_pos, n1, (int)symm, Bm);
         GMC_HIP(hipGetLastError());
         AbPair blocks{Bp, Bm, Bp, Bm, ab.cicoef, ab.ecasscf};
         const double zero = 0.0;

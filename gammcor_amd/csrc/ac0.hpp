@@ -25,7 +25,10 @@ size_t ac0_bytes(int n, int n1);
 // ECorr of AC0CAS from A(0), A(1) (ab) and the integrand weights Gw.  pl is
 // an FP64 ERPA plan of order n1 and batch 1 with a bound eigensolver;
 // erpa_scratch its workspace; eout holds n1 eigenvalues, one W and one nneg.
+// symm: AC0CAS_FOFO's form (abfofo.f90:1485), the alpha=0 blocks and A'
+// symmetrised whole instead of their block-order lower triangles.
 int ac0_correction(hipStream_t s, const AbPair& ab, const double* Gw, int n, const Ac0Blocks& b, ErpaPlan<double>* pl,
-                   void* erpa_scratch, const ErpaOut& eout, void* scratch, size_t scratch_bytes, double* ecorr);
+                   void* erpa_scratch, const ErpaOut& eout, void* scratch, size_t scratch_bytes, double* ecorr,
+                   bool symm = false);
 
 }  // namespace gmc

@@ -27,6 +27,6 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
                      double* eig_dev, int* nneg_host);
 
 // AC0CAS (acfd.f:1902) for a prepared system; indn_host is IndN(2, ndimx).
-int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr);
+int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr, bool symm = false);
 
 }  // namespace gmc

@@ -242,8 +242,9 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
     return 0;
 }
 
-// AC0CAS for a prepared system (A(0), A(1) and Gw assembled in r).
-int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr)
+// AC0CAS (AC0CAS_FOFO with symm) for a prepared system (A(0), A(1) and Gw
+// assembled in r).
+int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr, bool symm)
 {
     State& st = state();
     const int ndimx = r.sys.ndimx;
@@ -271,12 +272,40 @@ int ac0_run(hipStream_t s, AcRun& r, const int* indn_host, double* ecorr)
     out.W = reinterpret_cast<double*>(base + ews);
     out.eig = reinterpret_cast<double*>(base + ews + align_up(sizeof(double)));
     out.nneg = reinterpret_cast<int*>(base + ews + align_up(sizeof(double)) + align_up(n1e));
-    GMC_TRY(ac0_correction(s, r.ab, r.Gw, ndimx, b, &pl, base, out, base + ews + outb, acb, ecorr));
+    GMC_TRY(ac0_correction(s, r.ab, r.Gw, ndimx, b, &pl, base, out, base + ews + outb, acb, ecorr, symm));
     if (n1) GMC_TRY(eig_check_error<double>(s, *pl.eig));
     return 0;
 }
 
 }  // namespace gmc
+
+// The FOFO integral classes (FF|OO), (FO|FO) built from the packed AO
+// integrals into st.fofo, then A(0), A(1) and the integrand weights of the
+// system assembled from them (tran4_gen, tran.f90:313-423; AB_CAS_FOFO,
+// abfofo.f90:8-293).
+static int fofo_system_dev(hipStream_t s, int nbasis, int ninact, int nact, const double* ao, const double* uaomo,
+                           const double* occ, const int* igem, const double* xone, const double* rdm2act, int ndimx,
+                           const int* indn, AcRun& r)
+{
+    State& st = state();
+    st.sys_valid = false;   // acws and fofo are overwritten: drop the host-call caches
+    st.fofo_key[0] = st.fofo_key[1] = nullptr;
+    FofoPlan fp;
+    GMC_TRY(fofo_plan(nbasis, ninact + nact, &fp));
+    GMC_TRY(st.fofo.reserve(carve_bytes({fp.j_elems * 8, fp.k_elems * 8})));
+    Arena ar{static_cast<char*>(st.fofo.ptr), st.fofo.bytes};
+    double* J = ar.take<double>(fp.j_elems);
+    double* K = ar.take<double>(fp.k_elems);
+    GMC_CHECK(ar.fits(), "FOFO: layout overflow");
+    GMC_TRY(st.scratch.reserve(fp.scratch_bytes));
+    GMC_TRY(fofo_build(s, fp, ao, uaomo, J, K, st.scratch.ptr));
+    GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, nullptr, occ, igem, xone, nullptr, rdm2act, indn, r));
+    r.sys.J = J;
+    r.sys.K = K;
+    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
+    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    return 0;
+}
 
 template <class T>
 static int syevd_dev_impl(int n, int batch, T* A, T* w, hipStream_t s)
@@ -391,7 +420,27 @@ int gmc_ac0cas_dev(int nbasis, int ninact, int nact, const double* twono, const 
     GMC_HIP(hipMemcpyAsync(hind.data(), indn, hind.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     GMC_HIP(hipStreamSynchronize(s));
     double e = 0.0;
-    GMC_TRY(ac0_run(s, r, hind.data(), &e));
+    GMC_TRY(ac0_run(s, r, hind.data(), &e, false));
+    if (ecorr) *ecorr = e;
+    if (ecasscf) *ecasscf = r.ab.ecasscf;
+    return 0;
+}
+
+int gmc_ac0cas_fofo_dev(int nbasis, int ninact, int nact, const double* ao, const double* uaomo, const double* occ,
+                        const int* igem, const double* xone, const double* rdm2act, int ndimx, const int* indn,
+                        double* ecorr, double* ecasscf, void* stream)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(ndimx >= 1 && nbasis >= 2, "AC0CAS_FOFO: bad sizes");
+    GMC_CHECK(ninact + nact >= 1, "AC0CAS_FOFO: no occupied orbitals");
+    hipStream_t s = dev_stream(stream);
+    AcRun r;
+    GMC_TRY(fofo_system_dev(s, nbasis, ninact, nact, ao, uaomo, occ, igem, xone, rdm2act, ndimx, indn, r));
+    std::vector<int> hind(2 * (size_t)ndimx);
+    GMC_HIP(hipMemcpyAsync(hind.data(), indn, hind.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    GMC_HIP(hipStreamSynchronize(s));
+    double e = 0.0;
+    GMC_TRY(ac0_run(s, r, hind.data(), &e, true));
     if (ecorr) *ecorr = e;
     if (ecasscf) *ecasscf = r.ab.ecasscf;
     return 0;
@@ -407,25 +456,9 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
     GMC_CHECK(nalpha >= 1 && ndimx >= 1 && nbasis >= 2, "ACECORR FOFO: bad sizes");
     GMC_CHECK(ninact + nact >= 1, "ACECORR FOFO: no occupied orbitals");
     GMC_CHECK(prec >= GMC_FP64 && prec <= GMC_MIXED2, "ACECORR FOFO: unknown precision mode %d", prec);
-    State& st = state();
     hipStream_t s = dev_stream(stream);
-    st.sys_valid = false;   // acws and fofo are overwritten: drop the host-call caches
-    st.fofo_key[0] = st.fofo_key[1] = nullptr;
-    FofoPlan fp;
-    GMC_TRY(fofo_plan(nbasis, ninact + nact, &fp));
-    GMC_TRY(st.fofo.reserve(carve_bytes({fp.j_elems * 8, fp.k_elems * 8})));
-    Arena ar{static_cast<char*>(st.fofo.ptr), st.fofo.bytes};
-    double* J = ar.take<double>(fp.j_elems);
-    double* K = ar.take<double>(fp.k_elems);
-    GMC_CHECK(ar.fits(), "ACECORR FOFO: layout overflow");
-    GMC_TRY(st.scratch.reserve(fp.scratch_bytes));
-    GMC_TRY(fofo_build(s, fp, ao, uaomo, J, K, st.scratch.ptr));
     AcRun r;
-    GMC_TRY(ac_prepare(s, nbasis, ninact, nact, ndimx, nullptr, occ, igem, xone, nullptr, rdm2act, indn, r));
-    r.sys.J = J;
-    r.sys.K = K;
-    GMC_TRY(assemble_ab01(s, r.sys, r.ab_scratch, r.ab));
-    GMC_TRY(integrand_weights(s, r.sys, r.ab.cicoef, r.Gw, r.wdiag));
+    GMC_TRY(fofo_system_dev(s, nbasis, ninact, nact, ao, uaomo, occ, igem, xone, rdm2act, ndimx, indn, r));
     std::vector<double> w(nalpha);
     GMC_TRY(ac_alpha_batches(s, r, prec, alpha_batch > 0 ? alpha_batch : 16, nalpha, alphas, w.data(), eig, nneg));
     double e = 0.0;

@@ -485,6 +485,21 @@ int gmc_ac0cas(double* ecorr, double* etot, const double* twono, int64_t ninte2,
     return 0;
 }
 
+int gmc_ac0cas_fofo(double* ecorr, double* etot, const double* jints, const double* kints, const double* ure,
+                    const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
+                    int ndimx, int nbasis, int ninact, int nact)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(ndimx >= 1 && nbasis >= 2, "AC0CAS_FOFO: bad sizes");
+    hipStream_t s = state().stream;
+    AcRun* r = nullptr;
+    GMC_TRY(host_system_fofo(s, nbasis, ninact, nact, ndimx, jints, kints, occ, igem, xone, ure, rdm2act, indn,
+                             &r));
+    GMC_TRY(ac0_run(s, *r, indn, ecorr, true));
+    if (etot) *etot = r->ab.ecasscf;
+    return 0;
+}
+
 int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, const double* twono, int64_t ninte2,
                 const double* occ, const int* igem, const double* xone, const double* ure, const double* rdm2act,
                 int ndimx, const int* indn, int nalpha, const double* alphas, const double* weights, double* ecorr,

@@ -244,3 +244,41 @@ C     in the caller's object, or the module's copy removed).
       Deallocate(RDM2Act)
       Return
       End
+C
+C     AC0CAS_FOFO (abfofo.f90:1485-1750), the FOFO form of the linearised
+C     AC correction, on the GPU.  The two integral classes come from the
+C     record files IntJFile ((FF|OO)) and IntKFile ((FO|FO)) in the layout
+C     ACEInteg_FOFO reads, the active 2-RDM from rdm2.dat as ABPM0_FOFO
+C     reads it (abfofo.f90:1161-1168).  ECorr and ETot come back; the
+C     scratch arrays ABPLUS and ABMIN are not written.  Like ACEInteg_FOFO
+C     the reference defines it inside MODULE abfofo, so callers are bound to
+C     this external one at link time.
+      Subroutine AC0CAS_FOFO(ECorr,ETot,Occ,URe,XOne,ABPLUS,ABMIN,
+     $ IndN,IndX,IGemIN,NAct,INActive,NDimX,NBasis,NDim,NInte1,
+     $ NoSt,IntJFile,IntKFile)
+      use gammcor_hip
+      Implicit Real*8 (A-H,O-Z)
+      Character(*) IntJFile,IntKFile
+      Dimension Occ(NBasis),URe(NBasis,NBasis),XOne(NInte1),
+     $ ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),IndN(2,NDim),IndX(NDim),
+     $ IGemIN(NBasis)
+      Real*8, Allocatable :: RDM2Act(:),FFOO(:),FOFO(:)
+      Integer(8) NJ
+      If(NoSt.Ne.1) Then
+      Write(6,'(1X,A)') 'gammcor_hip: AC0CAS_FOFO supports NoSt=1'
+      Stop 1
+      EndIf
+      Call GmcCheckIndX(IndX,NDimX,NDim,'AC0CAS_FOFO')
+      NOcc=INActive+NAct
+      NJ=Int(NOcc,8)**2*Int(NBasis,8)**2
+      Allocate(FFOO(NJ),FOFO(NJ))
+      Call GmcReadRecords(IntJFile,FFOO,NJ)
+      Call GmcReadRecords(IntKFile,FOFO,NJ)
+      Write(6,'(1X,A)') 'gammcor_hip: AC0CAS_FOFO on the GPU'
+      Allocate(RDM2Act(Max(1,NAct**2*(NAct**2+1)/2)))
+      Call GmcRdm2Act(RDM2Act,NAct)
+      Call gmc_check(gmc_ac0cas_fofo(ECorr,ETot,FFOO,FOFO,URe,Occ,XOne,
+     $ IGemIN,RDM2Act,IndN,NDimX,NBasis,INActive,NAct),'AC0CAS_FOFO')
+      Deallocate(RDM2Act,FFOO,FOFO)
+      Return
+      End

@@ -6,7 +6,7 @@ module gammcor_hip
   private
   public :: GMC_FP64, GMC_FP32, GMC_MIXED, GMC_MIXED2
   public :: gmc_init, gmc_finalize, gmc_twono1, gmc_ab_cas, gmc_erpasymm1, gmc_aceneerpa, gmc_aceinteg, &
-            gmc_aceinteg_fofo, gmc_acecorr, gmc_ac0cas, gmc_forget_resident, gmc_check, gmc_prec_from_env
+            gmc_aceinteg_fofo, gmc_acecorr, gmc_ac0cas, gmc_ac0cas_fofo, gmc_forget_resident, gmc_check, gmc_prec_from_env
 
   integer(c_int), parameter :: GMC_FP64 = 0, GMC_FP32 = 1, GMC_MIXED = 2, GMC_MIXED2 = 3
 
@@ -83,6 +83,15 @@ module gammcor_hip
       real(c_double) :: ecorr, etot
       real(c_double), intent(in) :: twono(*), ure(*), occ(*), xone(*), rdm2act(*)
       integer(c_int64_t), value :: ninte2
+      integer(c_int), intent(in) :: igem(*), indn(*)
+      integer(c_int), value :: ndimx, nbasis, ninact, nact
+    end function
+
+    integer(c_int) function gmc_ac0cas_fofo(ecorr, etot, jints, kints, ure, occ, xone, igem, rdm2act, indn, ndimx, &
+                                            nbasis, ninact, nact) bind(C, name="gmc_ac0cas_fofo")
+      import :: c_int, c_double
+      real(c_double) :: ecorr, etot
+      real(c_double), intent(in) :: jints(*), kints(*), ure(*), occ(*), xone(*), rdm2act(*)
       integer(c_int), intent(in) :: igem(*), indn(*)
       integer(c_int), value :: ndimx, nbasis, ninact, nact
     end function

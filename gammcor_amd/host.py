@@ -153,6 +153,17 @@ def AC0CAS(dsys, stream=None):
     return dict(ecorr=float(ec[0]), ecasscf=float(es[0]))
 
 
+def AC0CAS_FOFO(ao, uaomo, dsys, stream=None):
+    """The FOFO form of the linearised AC correction (AC0CAS_FOFO,
+    abfofo.f90:1485-1750), FP64, from packed AO integrals ``ao`` and orbitals
+    ``uaomo`` as ACECORR_FOFO takes them.  Returns ecorr and ecasscf."""
+    ec, es = np.zeros(1), np.zeros(1)
+    _lib.call("gmc_ac0cas_fofo_dev", dsys.nbasis, dsys.ninact, dsys.nact, ao.data_ptr(), uaomo.data_ptr(),
+              dsys.occ.data_ptr(), dsys.igem.data_ptr(), dsys.xone.data_ptr(), dsys.rdm2act.data_ptr(), dsys.ndimx,
+              dsys.indn.data_ptr(), ec.ctypes.data, es.ctypes.data, stream)
+    return dict(ecorr=float(ec[0]), ecasscf=float(es[0]))
+
+
 def ACECORR_FOFO(ao, uaomo, dsys, alphas, weights, prec=GMC_FP64, alpha_batch=0, return_eig=False, stream=None):
     """The FOFO variant (ACEInteg_FOFO, abfofo.f90:3664): packed AO integrals
     ``ao`` and orbitals ``uaomo`` (URe(NO,AO), column-major, both cuda

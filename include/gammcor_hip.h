@@ -87,6 +87,16 @@ int gmc_ac0cas(double* ecorr, double* etot, const double* twono, int64_t ninte2,
                const double* xone, const int* igem, const double* rdm2act, const int* indn, int ndimx, int nbasis,
                int ninact, int nact);
 
+/* AC0CAS_FOFO(ECorr,ETot,Occ,URe,XOne,ABPLUS,ABMIN,IndN,IndX,IGemIN,NAct,
+ * INActive,NDimX,NBasis,NDim,NInte1,NoSt,IntJFile,IntKFile) -
+ * abfofo.f90:1485-1750: the FOFO form of the linearised AC correction, FP64
+ * (see gmc_ac0cas_fofo_dev), from the two integral classes in the record
+ * order of gmc_aceinteg_fofo.  ecorr = ECorr, etot = ETot (w/o ENuc); the
+ * reference's ABPLUS and ABMIN scratch is not written. */
+int gmc_ac0cas_fofo(double* ecorr, double* etot, const double* jints, const double* kints, const double* ure,
+                    const double* occ, const double* xone, const int* igem, const double* rdm2act, const int* indn,
+                    int ndimx, int nbasis, int ninact, int nact);
+
 /* ACEInteg(ECorr,TwoNO,URe,Occ,XOne,UNOAO,ABPLUS,ABMIN,EigVecR,Eig,EGOne,
  * NGOcc,Title,NBasis,NInte1,NInte2,NDim,NGem,IndAux,ACAlpha,IndN,IndX,NDimX)
  * - acfd.f:4832-4836, its CASSCF branch with NoSt = 1: AB_CAS -> ERPASYMM1 ->
@@ -166,6 +176,17 @@ int gmc_acecorr_dev(int prec, int alpha_batch, int nbasis, int ninact, int nact,
 int gmc_ac0cas_dev(int nbasis, int ninact, int nact, const double* twono, const double* occ, const int* igem,
                    const double* xone, const double* ure, const double* rdm2act, int ndimx, const int* indn,
                    double* ecorr, double* ecasscf, void* stream);
+
+/* AC0CAS_FOFO (abfofo.f90:1485-1750) on the device: the (FF|OO) and (FO|FO)
+ * classes built from the packed AO integrals ao with the orbitals uaomo as
+ * gmc_acecorr_fofo_dev builds them, then the linearised AC correction of
+ * gmc_ac0cas_dev with AC0CAS_FOFO's choice of matrix parts: each alpha=0
+ * block and A' symmetrised whole (ERPASYMM0, sq_symmetrize) where AC0CAS
+ * takes their lower triangles in block order.  The two reference routines
+ * differ for a non-stationary reference; this one matches AC0CAS_FOFO. */
+int gmc_ac0cas_fofo_dev(int nbasis, int ninact, int nact, const double* ao, const double* uaomo, const double* occ,
+                        const int* igem, const double* xone, const double* rdm2act, int ndimx, const int* indn,
+                        double* ecorr, double* ecasscf, void* stream);
 
 /* The FOFO variant of ACECORR (ACEInteg_FOFO, abfofo.f90:3664-3789, with
  * AB_CAS_FOFO 8-293 and ACEneERPA_FOFO 3791-3927): the integrals enter as

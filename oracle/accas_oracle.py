@@ -330,6 +330,71 @@ def erpasymm0(ap, am):
     return y, x, eig
 
 
+def ac0cas_fofo(sysd, twono, indn=None):
+    """AC0CAS_FOFO (abfofo.f90:1485-1750), the FOFO form of the linearised
+    AC correction, restated on its own terms.  It differs from AC0CAS in
+    which parts of the (not exactly symmetric) A(0) and A' it uses:
+
+    - A(0) = ABPM0_FOFO (abfofo.f90:1105-1320), the alpha=0 matrices over
+      all pairs; each block (active-active, active-inactive per inactive
+      orbital, virtual-active per virtual orbital; blocks of create_AC0_blocks
+      1105, 1374-1483) goes to ERPASYMM0 whole, which symmetrises it
+      (interpa.f:1525-1533) - AC0CAS uses the triangle at or below the
+      diagonal in block order instead;
+    - a virtual-inactive pair keeps omega = A+(0)_ii, X = Y = 1/sqrt(2)
+      (abfofo.f90:1598-1613, 1662-1681);
+    - A' = AB_CAS_FOFO(alpha=1, AB1) (abfofo.f90:1621-1623) is dA/dalpha of
+      the affine A(alpha), symmetrised by sq_symmetrize (abfofo.f90:268-272,
+      tran.f90:1303-1319): (A' + A'^T) / 2;
+    - X^T A'+ X, Y^T A'- Y (abpm_tran, tran.f90:620-729), D = 2 (Tp - Tm) /
+      (omega_m + omega_n) over nonzero omegas, P = Y D Y^T (ABPM_BACKTRAN,
+      abfofo.f90:2152-2258);
+    - ECorr = sum over pairs pq, rs of (C_p+C_q)(C_r+C_s) P_pq,rs (rs|pq)
+      from the FOFO records, less the terms whose four orbitals share a
+      geminal (abfofo.f90:1697-1735), the geminals fixed by index ranges
+      (1 inactive, 2 active, 3 virtual; abfofo.f90:1543-1552)."""
+    if indn is None:
+        indn = select_pairs(sysd["occ"], sysd["nele"])
+    args = (sysd["nbasis"], sysd["ninact"], sysd["nact"], sysd["occ"], sysd["xone"], twono, sysd["gamma_act"],
+            sysd["igem"], indn)
+    a0p, a0m, ecas, ci = ab_cas(*args, 0.0)
+    a1p, a1m, _, _ = ab_cas(*args, 1.0)
+    a0p, a0m, a1p, a1m = (np.asarray(x) for x in (a0p, a0m, a1p, a1m))
+    label, _ = ac0_block_order(indn, sysd["nbasis"], sysd["ninact"], sysd["nact"])
+    n = len(indn)
+    sym = lambda a: 0.5 * (a + a.T)
+    dp = sym(a1p - a0p)
+    dm = sym(a1m - a0m)
+    y = np.zeros((n, n))
+    x = np.zeros((n, n))
+    eig = np.zeros(n)
+    ni, nocc = sysd["ninact"], sysd["ninact"] + sysd["nact"]
+    p = indn[:, 0] - 1
+    q = indn[:, 1] - 1
+    for b in range(label.max() + 1):
+        m = np.nonzero(label == b)[0]
+        if len(m) == 1 and p[m[0]] >= nocc and q[m[0]] < ni:
+            eig[m[0]] = a0p[m[0], m[0]]
+            y[m[0], m[0]] = x[m[0], m[0]] = 1.0 / np.sqrt(2.0)
+            continue
+        yb, xb, eb = erpasymm0(a0p[np.ix_(m, m)], a0m[np.ix_(m, m)])
+        y[np.ix_(m, m)] = yb
+        x[np.ix_(m, m)] = xb
+        eig[m] = eb
+    tp = x.T @ dp @ x
+    tm = y.T @ dm @ y
+    nz = eig != 0.0
+    den = eig[:, None] + eig[None, :]
+    d = np.where(nz[:, None] & nz[None, :], 2.0 * (tp - tm) / np.where(den != 0.0, den, 1.0), 0.0)
+    pm = y @ d @ y.T
+    cij = ci[p] + ci[q]
+    gem = np.where(np.arange(sysd["nbasis"]) < ni, 1, np.where(np.arange(sysd["nbasis"]) < nocc, 2, 3))
+    same = (gem[p] == gem[q])[:, None] & (gem[p][:, None] == gem[p][None, :]) & (gem[p] == gem[q])[None, :]
+    g = pair_integrals(twono, indn)
+    ecorr = float(np.sum(np.where(same, 0.0, cij[:, None] * cij[None, :] * pm * g)))
+    return dict(ecorr=ecorr, ecasscf=ecas, eig=eig, ndimx=n)
+
+
 def ac0cas(sysd, twono, indn=None):
     """The linearised AC correction of AC0CAS (acfd.f:1902-2614).
 

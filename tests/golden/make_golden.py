@@ -53,6 +53,13 @@ CASES = {
     "n20_cas44_noinact_ac0": (dict(nbasis=20, ninact=0, nact=4, nelact=4, rank=50, sigma=0.1, seed=2), "ac0", 1),
     "n24_cas66_ac0": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "ac0", 1),
     "n48_cas44_ac0": (dict(synth.CONFIGS["n48_cas44"]), "ac0", 1),
+    # AC0CAS_FOFO (abfofo.f90:1485) on the FFOO/FOFO records of the same systems
+    "n12_cas44_ac0fofo": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "ac0fofo", 1),
+    "n16_cas22_ac0fofo": (dict(nbasis=16, ninact=3, nact=2, nelact=2, rank=30, sigma=0.1, seed=1), "ac0fofo", 1),
+    "n20_cas44_noinact_ac0fofo": (dict(nbasis=20, ninact=0, nact=4, nelact=4, rank=50, sigma=0.1, seed=2), "ac0fofo",
+                                  1),
+    "n24_cas66_ac0fofo": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "ac0fofo", 1),
+    "n48_cas44_ac0fofo": (dict(synth.CONFIGS["n48_cas44"]), "ac0fofo", 1),
 }
 SMALL = [k for k in CASES if not k.startswith(("n160", "n240"))]
 

@@ -90,3 +90,14 @@ def test_reference_ac0_driver_through_the_fortran_drop_in(cuda, name):
     assert "gammcor_hip: AC0CAS on the GPU" in out
     assert abs(res["ac0_ecorr"] - fx["ac0_ecorr"]) < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))
     assert abs(res["ecasscf"] - fx["ecasscf"]) < 1e-10
+
+
+@needs_driver
+@pytest.mark.parametrize("name", ["n16_cas22_ac0fofo", "n24_cas66_ac0fofo"])
+def test_reference_ac0fofo_driver_through_the_fortran_drop_in(cuda, name):
+    """The driver writes the FFOO and FOFO record files and calls
+    AC0CAS_FOFO (a procedure of MODULE abfofo, rebound to the external
+    drop-in at link time), which reads them and runs on the GPU."""
+    fx, res, out = _run(name, "ac0fofo")
+    assert "gammcor_hip: AC0CAS_FOFO on the GPU" in out
+    assert abs(res["ac0_ecorr"] - fx["ac0_ecorr"]) < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))

@@ -370,6 +370,81 @@ def test_ac0_alpha0_part_cancels(monkeypatch):
     monkeypatch.setattr(O, "erpasymm0", e0)
 
 
+# ---------------------------------------------------------------- AC0CAS_FOFO
+AC0FOFO = ["n12_cas44_ac0fofo", "n16_cas22_ac0fofo", "n20_cas44_noinact_ac0fofo", "n24_cas66_ac0fofo",
+           "n48_cas44_ac0fofo"]
+
+
+@pytest.mark.parametrize("name", AC0FOFO)
+def test_ac0fofo_matches_reference(name):
+    """The FOFO form of the linearised AC correction against the
+    reference's own AC0CAS_FOFO (abfofo.f90:1485) on the FFOO/FOFO records
+    of the same system."""
+    import accas_oracle as O
+    fx = gold(name)
+    s = system_of(fx)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    r = O.ac0cas_fofo(O.system_dict(s), tw)
+    assert r["ndimx"] == fx["ndimx"]
+    assert abs(r["ecasscf"] - fx["ecasscf"]) < 1e-10
+    assert abs(r["ecorr"] - fx["ac0_ecorr"]) < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))
+
+
+def test_ac0fofo_differs_from_ac0cas_by_the_triangle():
+    """AC0CAS_FOFO and AC0CAS are not two routes to one number: AC0CAS takes
+    the triangle at or below the diagonal (in block order) of the alpha=0
+    blocks and of A', AC0CAS_FOFO their symmetrised whole.  The two
+    reference routines differ by more than 1e-3 Eh on every system, and
+    each restatement matches only its own."""
+    import accas_oracle as O
+    for name in ("n12_cas44", "n24_cas66"):
+        ff, fa = gold(name + "_ac0fofo"), gold(name + "_ac0")
+        s = system_of(ff)
+        tw = O.two_no1(s.twoel_ao, s.uaomo)
+        sd = O.system_dict(s)
+        rf, ra = O.ac0cas_fofo(sd, tw)["ecorr"], O.ac0cas(sd, tw)["ecorr"]
+        assert abs(ff["ac0_ecorr"] - fa["ac0_ecorr"]) > 1e-3
+        assert abs(rf - ff["ac0_ecorr"]) < 1e-10 * max(1.0, abs(rf))
+        assert abs(ra - ff["ac0_ecorr"]) > 1e-3
+
+
+def test_ac0fofo_planted_mistakes_are_caught(monkeypatch):
+    """X and Y of the blocks swapped, all pairs in one alpha=0 block, and
+    the unsymmetrised A' (its upper triangle ignored by sq_symmetrize's
+    mirror) each break the match with the reference's AC0CAS_FOFO."""
+    import accas_oracle as O
+    fx = gold("n24_cas66_ac0fofo")
+    s = system_of(fx)
+    tw = O.two_no1(s.twoel_ao, s.uaomo)
+    sd = O.system_dict(s)
+
+    def err():
+        return abs(O.ac0cas_fofo(sd, tw)["ecorr"] - fx["ac0_ecorr"])
+
+    assert err() < 1e-10 * max(1.0, abs(fx["ac0_ecorr"]))
+    e0 = O.erpasymm0
+    monkeypatch.setattr(O, "erpasymm0", lambda ap, am: (lambda y, x, e: (x, y, e))(*e0(ap, am)))
+    assert err() > 1e-6
+    monkeypatch.setattr(O, "erpasymm0", e0)
+    blocks = O.ac0_block_order
+    monkeypatch.setattr(O, "ac0_block_order", lambda *a: (np.zeros(len(a[0]), dtype=np.int64), blocks(*a)[1]))
+    assert err() > 1e-6
+    monkeypatch.setattr(O, "ac0_block_order", blocks)
+    orig = O.ab_cas
+
+    def skewed(*a, **k):
+        # alpha = 1 with its strict upper triangle doubled: a symmetrised A'
+        # moves, the lower triangle alone would not
+        ap, am, e, c = orig(*a, **k)
+        if a[-1] == 1.0:
+            ap = np.asarray(ap) + np.triu(np.asarray(ap), 1)
+            am = np.asarray(am) + np.triu(np.asarray(am), 1)
+        return ap, am, e, c
+
+    monkeypatch.setattr(O, "ab_cas", skewed)
+    assert err() > 1e-6
+
+
 # ---------------------------------------------------------------- rdm2.dat
 def test_rdm2_dat_round_trip():
     from gammcor_amd import synth, host
