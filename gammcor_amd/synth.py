@@ -280,10 +280,20 @@ def unpack_sym(v, n):
     return m
 
 
-def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=True, ci_ref_weight=None, **_):
+def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=True, ci_ref_weight=None,
+                energy_scale=1.0, occ_act_set=(), degenerate_active=None, **_):
     """One seeded system.  ci_ref_weight: |c|^2 put on the leading determinant
     of the CI vector (None: CI_REF_WEIGHT; 0: a plain normalised Gaussian
-    vector over all determinants)."""
+    vector over all determinants).
+
+    Edge cases of the reference's branches (the RDMs stay those of the CI
+    vector; reference and library read the same inputs):
+      energy_scale: h and the two-electron integrals times this factor (the
+        Cholesky factors times its square root), so every ERPA omega scales
+        by it - used to put omegas on either side of the 1e-3 filter;
+      occ_act_set: (k, value) pairs overriding active occupation k;
+      degenerate_active: (k1, k2) gives active orbital k2 the occupation of
+        k1, a pair the pair selection discards (accas.f:67-71)."""
     if ci_ref_weight is None:
         ci_ref_weight = CI_REF_WEIGHT
     s_l, s_c, s_ci, s_h = seed * 10 + 0, seed * 10 + 1, seed * 10 + 2, seed * 10 + 3
@@ -307,6 +317,13 @@ def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=
     occ[ninact:ninact + nact] = occ_act
     chol = cholesky_factors(nbasis, rank, sigma, s_l)
     h = one_electron(nbasis, ninact, nact, s_h, mean_field(chol, c_no.T, occ))
+    for k, v in occ_act_set:
+        occ[ninact + int(k)] = float(v)
+    if degenerate_active is not None:
+        occ[ninact + int(degenerate_active[1])] = occ[ninact + int(degenerate_active[0])]
+    if energy_scale != 1.0:
+        h = h * energy_scale
+        chol = chol * math.sqrt(energy_scale)
     sysm = System(nbasis=nbasis, ninact=ninact, nact=nact, nelact=nelact, occ=occ,
                   xone=pack_sym(h), uaomo=np.ascontiguousarray(c_no.T), rdm2_text=text,
                   gamma_act=gamma_read)
@@ -314,6 +331,12 @@ def make_system(nbasis, ninact, nact, nelact, rank, sigma, seed, with_integrals=
                      seed=seed, seeds=dict(chol=s_l, orbitals=s_c, ci=s_ci, hcore=s_h),
                      h_sigma=H_SIGMA, gap=GAP, widths=[W_INACT, W_ACT, W_VIRT],
                      ci_ref_weight=ci_ref_weight, enuc=0.0)
+    if energy_scale != 1.0:
+        sysm.meta["energy_scale"] = energy_scale
+    if occ_act_set:
+        sysm.meta["occ_act_set"] = [[int(k), float(v)] for k, v in occ_act_set]
+    if degenerate_active is not None:
+        sysm.meta["degenerate_active"] = [int(x) for x in degenerate_active]
     sysm.chol = chol
     if with_integrals:
         sysm.twoel_ao = ao_integrals_packed(chol)
@@ -372,6 +395,13 @@ def ao_integrals_device(chol, device):
         mask = cols[None, :r1] <= rows
         ao[r0 * (r0 + 1) // 2: r1 * (r1 + 1) // 2] = block[mask]
     return ao
+
+
+def system_from_meta(meta, with_integrals=True):
+    """The system a fixture records (its `system` entry)."""
+    keys = ("nbasis", "ninact", "nact", "nelact", "rank", "sigma", "seed", "ci_ref_weight", "energy_scale",
+            "occ_act_set", "degenerate_active")
+    return make_system(with_integrals=with_integrals, **{k: meta[k] for k in keys if k in meta})
 
 
 def config_system(name, with_integrals=True):

@@ -53,6 +53,21 @@ CASES = {
     "n20_cas44_noinact_ac0": (dict(nbasis=20, ninact=0, nact=4, nelact=4, rank=50, sigma=0.1, seed=2), "ac0", 1),
     "n24_cas66_ac0": (dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed=5), "ac0", 1),
     "n48_cas44_ac0": (dict(synth.CONFIGS["n48_cas44"]), "ac0", 1),
+    # edge cases of the reference's branches on the NBasis=12 system:
+    # a nearly degenerate active pair discarded (accas.f:67-71); an active
+    # occupation of exactly 1.0 (the Occ.Eq.One branches, interpa.f:3926,
+    # 3962, 3999, 4028) against 1 - 1e-12 (the active branches); the
+    # Hamiltonian scaled so the smallest omega lies just above 1e-3 at three
+    # nodes and below it (omega^2 <= 1e-6: left unnormalised and filtered
+    # out, interpa.f:1748-1776, acfd.f:4976, 5013) at the fourth
+    "n12_cas44_degen": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0,
+                             degenerate_active=(1, 2)), "grid", 4),
+    "n12_cas44_occ1": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0,
+                            occ_act_set=[(0, 1.0)]), "grid", 4),
+    "n12_cas44_occ1eps": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0,
+                               occ_act_set=[(0, 1.0 - 1e-12)]), "grid", 4),
+    "n12_cas44_omega_edge": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0,
+                                  energy_scale=1.2823e-3), "grid", 4),
     # AC0CAS_FOFO (abfofo.f90:1485) on the FFOO/FOFO records of the same systems
     "n12_cas44_ac0fofo": (dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed=0), "ac0fofo", 1),
     "n16_cas22_ac0fofo": (dict(nbasis=16, ninact=3, nact=2, nelact=2, rank=30, sigma=0.1, seed=1), "ac0fofo", 1),
@@ -102,6 +117,10 @@ def parse(out):
 def run_case(name):
     args, mode, nal = CASES[name]
     s = synth.make_system(**args)
+    if "occ_act_set" in args:
+        # the occupations reach the driver as binary doubles: 1 - 1e-12 stays
+        # apart from 1.0
+        assert all(s.occ[s.ninact + k] == v for k, v in args["occ_act_set"])
     alphas, weights = synth.gauss_legendre_01(nal)
     with tempfile.TemporaryDirectory() as d:
         synth.write_inputs(s, d, alphas, weights)
