@@ -309,6 +309,37 @@ def ACECORR_host(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2ac
     return dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
 
 
+# ---------------------------------------------------------------- SAPT
+def SAPT_RESPONSE(dsys, stream=None):
+    """A monomer's ERPA response for SAPT (sapt_main.f90:2593-2714): AB_CAS at
+    alpha = 1, ERPASYMMXY, EigVecR = (C_p - C_q)(EigY - EigX).  Returns a
+    dict of device tensors evec (ndimx x ndimx, column-major: evec[k] is
+    column k), eig and cicoef."""
+    import torch
+    dev = dsys.occ.device
+    n = dsys.ndimx
+    evec = torch.zeros(n * n, dtype=torch.float64, device=dev)
+    eig = torch.zeros(n, dtype=torch.float64, device=dev)
+    cic = torch.zeros(dsys.nbasis, dtype=torch.float64, device=dev)
+    ure = dsys.ure.data_ptr() if dsys.ure is not None else None
+    _lib.call("gmc_sapt_response_dev", dsys.nbasis, dsys.ninact, dsys.nact, dsys.twono.data_ptr(),
+              dsys.occ.data_ptr(), dsys.igem.data_ptr(), dsys.xone.data_ptr(), ure, dsys.rdm2act.data_ptr(), n,
+              dsys.indn.data_ptr(), evec.data_ptr(), eig.data_ptr(), cic.data_ptr(), stream)
+    return dict(evec=evec, eig=eig, cicoef=cic, indn=dsys.indn, ndimx=n, nbasis=dsys.nbasis)
+
+
+def E2DISP(ra, rb, vab, stream=None):
+    """Coupled E2disp (sapt_pol.f90:1684-2034) of two monomer responses and
+    the intermolecular pair integrals vab (device, ndimx_A x ndimx_B,
+    column-major, (pq|rs)_AB over the two pair lists).  Hartree."""
+    e = np.zeros(1)
+    _lib.call("gmc_e2disp_dev", ra["nbasis"], ra["ndimx"], ra["indn"].data_ptr(), ra["cicoef"].data_ptr(),
+              ra["evec"].data_ptr(), ra["eig"].data_ptr(), rb["ndimx"], rb["indn"].data_ptr(),
+              rb["cicoef"].data_ptr(), rb["evec"].data_ptr(), rb["eig"].data_ptr(), vab.data_ptr(), e.ctypes.data,
+              stream)
+    return float(e[0])
+
+
 # ---------------------------------------------------------------- input files
 def _buf(n):
     return np.zeros(n, dtype=np.float64)

@@ -416,12 +416,13 @@ def gauss_legendre_01(n):
     return 0.5 * (x + 1.0), 0.5 * w
 
 
-def write_inputs(sysm, directory, alphas, weights):
+def write_inputs(sysm, directory, alphas, weights, tag=""):
     """Write a system in the binary layout read by the reference driver
-    (oracle/ref_driver.f90): little-endian int32 header then float64 arrays."""
+    (oracle/ref_driver.f90): little-endian int32 header then float64 arrays.
+    tag "_A" / "_B" names a monomer's files (inputs_A.bin, rdm2_A.dat)."""
     os.makedirs(directory, exist_ok=True)
     hdr = np.array([sysm.nbasis, sysm.ninact, sysm.nact, sysm.nele, len(alphas)], dtype=np.int32)
-    with open(os.path.join(directory, "inputs.bin"), "wb") as f:
+    with open(os.path.join(directory, f"inputs{tag}.bin"), "wb") as f:
         f.write(hdr.tobytes())
         f.write(np.ascontiguousarray(sysm.occ, dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(sysm.xone, dtype=np.float64).tobytes())
@@ -431,5 +432,43 @@ def write_inputs(sysm, directory, alphas, weights):
         f.write(np.asarray(weights, dtype=np.float64).tobytes())
     with open(os.path.join(directory, "twoel_ao.bin"), "wb") as f:
         f.write(np.ascontiguousarray(sysm.twoel_ao, dtype=np.float64).tobytes())
-    with open(os.path.join(directory, "rdm2.dat"), "w") as f:
+    with open(os.path.join(directory, f"rdm2{tag}.dat"), "w") as f:
         f.write(sysm.rdm2_text)
+
+
+def dimer(nbasis, ninact, nact, nelact, rank, sigma, seed_a, seed_b):
+    """Two monomers in one basis for SAPT: A and B each from its own seed
+    (orbitals, CI vector, h), both with the AO integrals of A's Cholesky
+    factors.  Returns (A, B, vab) with the intermolecular NO integrals
+    vab[p, q, r, s] = (pq|rs)_AB = sum_P (U_A L_P U_A^T)_pq (U_B L_P U_B^T)_rs,
+    p, q in A's orbitals and r, s in B's."""
+    a = make_system(nbasis, ninact, nact, nelact, rank, sigma, seed_a)
+    b = make_system(nbasis, ninact, nact, nelact, rank, sigma, seed_b, with_integrals=False)
+    b.twoel_ao, b.chol = a.twoel_ao, a.chol
+    b.meta["shares_ao_of_seed"] = seed_a
+    il, jl = np.tril_indices(nbasis)
+    lp = np.zeros((a.chol.shape[0], nbasis, nbasis))
+    lp[:, il, jl] = a.chol
+    lp[:, jl, il] = a.chol
+    ba = np.matmul(np.matmul(a.uaomo[None], lp), a.uaomo.T[None])
+    bb = np.matmul(np.matmul(b.uaomo[None], lp), b.uaomo.T[None])
+    vab = np.einsum("Ppq,Prs->pqrs", ba, bb, optimize=True)
+    return a, b, vab
+
+
+def twomoab_records(vab, a, b):
+    """The TWOMOAB direct-access records e2disp reads (sapt_pol.f90:1652-1664):
+    record iq + (ip - num0_A - 1) dimO_A (1-based ip > num0_A, iq <= dimO_A)
+    holds, at is + (ir - num0_B - 1) dimO_B, (ip iq|ir is)_AB; num0 = inactive
+    orbitals, dimO = inactive + active."""
+    n = a.nbasis
+    n0a, doa = a.ninact, a.ninact + a.nact
+    n0b, dob = b.ninact, b.ninact + b.nact
+    nova, novb = doa * (n - n0a), dob * (n - n0b)
+    out = np.zeros((nova, novb))
+    for ip in range(n0a + 1, n + 1):
+        for iq in range(1, doa + 1):
+            rec = iq + (ip - n0a - 1) * doa
+            blk = vab[ip - 1, iq - 1, n0b:, :dob]          # [ir - n0b - 1, is - 1]
+            out[rec - 1, :] = blk.reshape(-1)                # is fastest
+    return out

@@ -208,6 +208,32 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
  * Replaces the DSYEVD behind Diag8 (misc.f:541-616) without its transpose. */
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
 
+/* ---- SAPT: second-order dispersion from the same A+-B and ERPA ----------- */
+
+/* A monomer's ERPA response as SAPT computes it (sapt_main.f90:2593-2714):
+ * AB_CAS at alpha = 1, ERPASYMMXY (interpa.f:1401) and the response vectors
+ * EigVecR = (C_p - C_q)(EigY - EigX) that writeresp stores (4068-4080),
+ * FP64.  Device arrays as gmc_acecorr_dev takes them; out: evec
+ * (ndimx x ndimx, column k the response of mode k, zero where
+ * omega^2 <= 1e-6), eig (ndimx: omega, or omega^2 where <= 1e-6, ascending)
+ * and cicoef (nbasis, may be NULL: the CICoef of the monomer). */
+int gmc_sapt_response_dev(int nbasis, int ninact, int nact, const double* twono, const double* occ, const int* igem,
+                          const double* xone, const double* ure, const double* rdm2act, int ndimx, const int* indn,
+                          double* evec, double* eig, double* cicoef, void* stream);
+
+/* e2disp (sapt_pol.f90:1684-2034), the coupled E2disp of monomers A and B
+ * in one basis from their responses (gmc_sapt_response_dev) and the
+ * intermolecular integrals over their pairs, vab (ndimxa x ndimxb,
+ * column-major) = (pq|rs)_AB with pq = IndN_A(:, i), rs = IndN_B(:, j) (the
+ * values the reference reads from its TWOMOAB records):
+ *   E2disp = -16 sum_ij [Y_A^T diag(C_p+C_q) V diag(C_r+C_s) Y_B]_ij^2
+ *                 / (omegaA_i + omegaB_j),   1e-3 < |omega| < 1e8.
+ * Device arrays; e2disp is a host scalar (Hartree, not the reference's
+ * printed milli-Hartree). */
+int gmc_e2disp_dev(int nbasis, int ndimxa, const int* indna, const double* cicoefa, const double* eveca,
+                   const double* eiga, int ndimxb, const int* indnb, const double* cicoefb, const double* evecb,
+                   const double* eigb, const double* vab, double* e2disp, void* stream);
+
 /* ---- readers of the reference's input files (host code; no GPU) --------
  * They return the arrays the entry points above take. */
 

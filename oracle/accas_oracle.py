@@ -449,6 +449,57 @@ def ac0cas(sysd, twono, indn=None):
     return dict(ecorr=ecorr, ecasscf=ecas, eig=eig, ndimx=n)
 
 
+def erpasymmxy(ap, am, cicoef, indn):
+    """ERPASYMMXY (interpa.f:1401-1446): ERPASYMM0's Y and X turned into the
+    X, Y of the original ERPA problem, X' = X / (C_p + C_q),
+    Y' = Y / (C_p - C_q), EigX = (X' - Y') / 2, EigY = (X' + Y') / 2.
+    Returns (EigY, EigX, Eig)."""
+    y0, x0, eig = erpasymm0(ap, am)
+    p = indn[:, 0] - 1
+    q = indn[:, 1] - 1
+    xs = x0 / (cicoef[p] + cicoef[q])[:, None]
+    ys = y0 / (cicoef[p] - cicoef[q])[:, None]
+    return 0.5 * (xs + ys), 0.5 * (xs - ys), eig
+
+
+def sapt_response(sysd, twono, indn=None):
+    """One monomer's response for SAPT (sapt_main.f90:2604-2660): A+-B at
+    alpha = 1, ERPASYMMXY, and the response vectors
+    EigVecR = (C_p - C_q)(EigY - EigX) with the omegas (what writeresp
+    stores, 4068-4080)."""
+    if indn is None:
+        indn = select_pairs(sysd["occ"], sysd["nele"])
+    ap, am, ecas, c = ab_cas(sysd["nbasis"], sysd["ninact"], sysd["nact"], sysd["occ"], sysd["xone"], twono,
+                             sysd["gamma_act"], sysd["igem"], indn, 1.0)
+    ey, ex, eig = erpasymmxy(np.asarray(ap), np.asarray(am), c, indn)
+    p = indn[:, 0] - 1
+    q = indn[:, 1] - 1
+    evec = (c[p] - c[q])[:, None] * (ey - ex)
+    return dict(evec=evec, eig=eig, cicoef=c, indn=indn, ecasscf=ecas)
+
+
+def e2disp(ra, rb, vab):
+    """The coupled second-order dispersion energy (e2disp,
+    sapt_pol.f90:1684-2034; the loops of its ICASSCF = 0, ISERPA = 0 branch,
+    1876-1907, are the same sum as the CAS branch's e2disp_intermed1):
+      tmp1(i, rs) = sum_pq (C_q+C_p)^A (C_s+C_r)^B (pq|rs) EVecA(pq, i)
+      tmp2 = tmp1 EVecB,
+      E2disp = -16 sum_ij tmp2_ij^2 / (omegaA_i + omegaB_j)
+    over 1e-3 < |omega| < 1e8 on both sides (1950-1966).  vab[p,q,r,s] holds
+    (pq|rs)_AB over A's and B's orbitals."""
+    pa, qa = ra["indn"][:, 0] - 1, ra["indn"][:, 1] - 1
+    pb, qb = rb["indn"][:, 0] - 1, rb["indn"][:, 1] - 1
+    ca, cb = ra["cicoef"], rb["cicoef"]
+    v = vab[pa[:, None], qa[:, None], pb[None, :], qb[None, :]]
+    fact = (ca[qa] + ca[pa])[:, None] * (cb[qb] + cb[pb])[None, :] * v
+    tmp2 = (ra["evec"].T @ fact) @ rb["evec"]
+    oa, ob = ra["eig"], rb["eig"]
+    conda = (np.abs(oa) > 1e-3) & (np.abs(oa) < 1e8)
+    condb = (np.abs(ob) > 1e-3) & (np.abs(ob) < 1e8)
+    t = tmp2[np.ix_(conda, condb)]
+    return -16.0 * float(np.sum(t * t / (oa[conda][:, None] + ob[condb][None, :])))
+
+
 def system_dict(sysm):
     return dict(nbasis=sysm.nbasis, ninact=sysm.ninact, nact=sysm.nact, occ=sysm.occ, xone=sysm.xone,
                 gamma_act=sysm.gamma_act, igem=sysm.igem, nele=sysm.nele)

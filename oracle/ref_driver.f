@@ -18,6 +18,14 @@ C       ac0   : AC0CAS (acfd.f:1902), the linearised AC correction, printing
 C               its ECorr and the CASSCF energy;
 C       ac0fofo: its FOFO version AC0CAS_FOFO (abfofo.f90:1485) on the FFOO
 C               and FOFO record files written as fofo does;
+C       e2disp: SAPT's second-order dispersion energy of two monomers in
+C               one basis (sapt_main.f90:2593-2714, sapt_pol.f90:1684-2034):
+C               per monomer (inputs_A.bin / inputs_B.bin with rdm2_A.dat /
+C               rdm2_B.dat, the AO integrals of twoel_ao.bin) its TwoNO1,
+C               AB_CAS at alpha = 1, ERPASYMMXY and the response vectors
+C               (C_p-C_q)(Y-X) written as PROP_A / PROP_B; the
+C               intermolecular integrals come as the TWOMOAB records; then
+C               the reference's e2disp, printing SAPT%e2disp;
 C       formats: the reference's own readers of its input files on files
 C               written by gammcor_amd.formats (no transform, no solve):
 C               readtwoint with the Molpro source (sorter.f90:31) writes
@@ -44,6 +52,10 @@ C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
       type(SystemBlock) :: System
 C
       Call Get_Command_Argument(1,Mode)
+      If(Mode(1:6).Eq.'e2disp') Then
+      Call RefE2Disp
+      Stop
+      EndIf
       Title='synthetic'
       Open(11,File='inputs.bin',Access='STREAM',Form='UNFORMATTED',
      $ Status='OLD')
@@ -302,4 +314,153 @@ C     results written as raw doubles for the test to compare.
       Write(20) XMone
       Close(20)
       Write(6,'(A,I4)') 'FORMATS_OK ',IWarn
+      End
+
+      Subroutine RefE2Disp
+C     Two monomers through the reference's SAPT response chain and its
+C     E2disp (sapt_pol.f90:1684).  The branch of e2disp for
+C     ICASSCF = 0, ISERPA = 0 forms the same coupled sum by plain loops
+C     and needs no alpha=0 response files (its uncoupled part is skipped).
+      use types
+      use sapt_pol
+      Implicit Real*8 (A-H,O-Z)
+      Include 'commons.inc'
+      type(SystemBlock) :: Mon(2)
+      type(FlagsData) :: Flags
+      type(SaptData) :: SAPT
+      Character*1 Tag(2)
+      Character*80 Line
+      Integer Hdr(5)
+      Real*8, Allocatable :: Occ(:),XOne(:),UAOMO(:,:),URe(:,:),
+     $ AO(:),TwoNO(:),ABPLUS(:),ABMIN(:),EigY(:),EigX(:),Eig(:),
+     $ EVec(:),AG(:),WG(:)
+      Integer, Allocatable :: IndN(:,:),IndX(:),IndAux(:)
+      Data Tag /'A','B'/
+      Do IM=1,2
+      Open(11,File='inputs_'//Tag(IM)//'.bin',Access='STREAM',
+     $ Form='UNFORMATTED',Status='OLD')
+      Read(11) Hdr
+      NBasis=Hdr(1)
+      NInAc=Hdr(2)
+      NAct=Hdr(3)
+      NEL=Hdr(4)
+      NAl=Hdr(5)
+      NInte1=NBasis*(NBasis+1)/2
+      NInte2=NInte1*(NInte1+1)/2
+      Allocate(Occ(NBasis),XOne(NInte1),UAOMO(NBasis,NBasis),
+     $ URe(NBasis,NBasis),AG(NAl),WG(NAl),TwoNO(NInte2))
+      Read(11) Occ
+      Read(11) XOne
+      Read(11) UAOMO
+      Read(11) AG
+      Read(11) WG
+      Close(11)
+      Open(12,File='twoel_ao.bin',Access='STREAM',Form='UNFORMATTED',
+     $ Status='OLD')
+      Read(12) TwoNO
+      Close(12)
+C     this monomer's rdm2.dat, where AB_CAS reads it
+      Open(13,File='rdm2_'//Tag(IM)//'.dat',Status='OLD')
+      Open(14,File='rdm2.dat',Status='REPLACE')
+   10 Read(13,'(A)',End=20) Line
+      Write(14,'(A)') Trim(Line)
+      GoTo 10
+   20 Close(13)
+      Close(14)
+      XELE=Float(NEL)
+      NELE=NEL
+      IFunSR=0
+      IFunSRKer=0
+      ITwoEl=1
+      ICASSCF=1
+      NAcCAS=NAct
+      NInAcCAS=NInAc
+      NoSt=1
+      ThrSelAct=1.D-8
+      ThrQVirt=1.D-7
+      Do I=1,NBasis
+      IGem(I)=3
+      If(I.Le.NInAc+NAct) IGem(I)=2
+      If(I.Le.NInAc) IGem(I)=1
+      EndDo
+      If(NInAc.Eq.0) Then
+      Do I=1,NBasis
+      IGem(I)=2
+      If(I.Le.NAct) IGem(I)=1
+      EndDo
+      EndIf
+      Call TwoNO1(TwoNO,UAOMO,NBasis,NInte2)
+      URe=0.D0
+      Do I=1,NBasis
+      URe(I,I)=1.D0
+      EndDo
+C     pair selection as the grid mode (accas.f:30-99)
+      NDim=NBasis*(NBasis-1)/2
+      Allocate(IndN(2,NDim),IndX(NDim),IndAux(NBasis))
+      Do I=1,NBasis
+      IndAux(I)=2
+      If(I.Le.NELE) IndAux(I)=0
+      If(Occ(I).Lt.1.D0.And.Occ(I).Ne.0.D0) IndAux(I)=1
+      EndDo
+      Ind=0
+      Do I=1,NBasis
+      Do J=1,I-1
+      IS=IndAux(I)+IndAux(J)
+      If(IS.Ne.0.And.IS.Ne.4) Then
+      If(.Not.(IndAux(I).Eq.1.And.IndAux(J).Eq.1.And.
+     $ Abs(Occ(I)-Occ(J))/Occ(I).Lt.ThrSelAct)) Then
+      If(Abs(Occ(I)+Occ(J)-2.D0).Gt.1.D-10.And.
+     $ Abs(Occ(I)+Occ(J)).Gt.ThrQVirt) Then
+      Ind=Ind+1
+      IndX(Ind)=Ind
+      IndN(1,Ind)=I
+      IndN(2,Ind)=J
+      EndIf
+      EndIf
+      EndIf
+      EndDo
+      EndDo
+      NDimX=Ind
+      Allocate(ABPLUS(NDimX*NDimX),ABMIN(NDimX*NDimX),
+     $ EigY(NDimX*NDimX),EigX(NDimX*NDimX),Eig(NDimX),
+     $ EVec(NDimX*NDimX))
+C     sapt_main.f90:2604-2660: A+-B at alpha = 1, ERPASYMMXY, and
+C     EigVecR = (C_p - C_q)(Y - X)
+      Call AB_CAS(ABPLUS,ABMIN,ETot,URe,Occ,XOne,TwoNO,IndAux,
+     $ IndN,IndX,NDimX,NBasis,NDimX,NInte1,NInte2,1.D0)
+      Call ERPASYMMXY(EigY,EigX,Eig,ABPLUS,ABMIN,Occ,IndN,NDimX,
+     $ NBasis)
+      Do J=1,NDimX
+      Do I=1,NDimX
+      IP=IndN(1,I)
+      IQ=IndN(2,I)
+      EVec((J-1)*NDimX+I)=(CICoef(IP)-CICoef(IQ))
+     $ *(EigY((J-1)*NDimX+I)-EigX((J-1)*NDimX+I))
+      EndDo
+      EndDo
+C     the response file as writeresp writes it (sapt_main.f90:4068)
+      Open(15,File='PROP_'//Tag(IM),Form='UNFORMATTED',
+     $ Status='REPLACE')
+      Write(15) EVec
+      Write(15) Eig
+      Close(15)
+      Mon(IM)%NBasis=NBasis
+      Mon(IM)%NDimX=NDimX
+      Mon(IM)%num0=NInAc
+      Mon(IM)%num1=NAct
+      Mon(IM)%num2=NBasis-NInAc-NAct
+      Allocate(Mon(IM)%IndN(2,NDimX),Mon(IM)%CICoef(NBasis))
+      Mon(IM)%IndN(1:2,1:NDimX)=IndN(1:2,1:NDimX)
+      Mon(IM)%CICoef(1:NBasis)=CICoef(1:NBasis)
+      Write(6,'(A,A,I8,ES25.17)') 'MONOMER ',Tag(IM),NDimX,ETot
+      Deallocate(Occ,XOne,UAOMO,URe,AG,WG,TwoNO,IndN,IndX,IndAux,
+     $ ABPLUS,ABMIN,EigY,EigX,Eig,EVec)
+      EndDo
+      Flags%ICASSCF=0
+      Flags%ISERPA=0
+      SAPT%SemiCoupled=.False.
+      SAPT%Wexcit=.False.
+      SAPT%IPrint=0
+      Call e2disp(Flags,Mon(1),Mon(2),SAPT)
+      Write(6,'(A,ES25.17)') 'E2DISP ',SAPT%e2disp
       End

@@ -13,6 +13,7 @@ stored: they are regenerated from the seeds recorded in each fixture.
     python tests/golden/make_golden.py small      # N <= 48 cases (seconds)
     python tests/golden/make_golden.py n160       # NBasis=160 config (minutes)
     python tests/golden/make_golden.py n240       # NBasis=240 config (about an hour, 25 GB)
+    python tests/golden/make_golden.py dimers     # SAPT E2disp of the seeded dimers
 """
 from __future__ import annotations
 
@@ -78,6 +79,38 @@ CASES = {
 }
 SMALL = [k for k in CASES if not k.startswith(("n160", "n240"))]
 
+# SAPT dimers (mode e2disp): name -> (dimer args of synth.dimer)
+DIMERS = {
+    "d12_cas44_e2disp": dict(nbasis=12, ninact=2, nact=4, nelact=4, rank=36, sigma=0.1, seed_a=0, seed_b=7),
+    "d24_cas66_e2disp": dict(nbasis=24, ninact=3, nact=6, nelact=6, rank=60, sigma=0.1, seed_a=5, seed_b=8),
+}
+
+
+def run_dimer(name):
+    """Two monomers through the reference's SAPT response chain and its
+    e2disp (driver mode e2disp)."""
+    args = DIMERS[name]
+    a, b, vab = synth.dimer(**args)
+    alphas, weights = synth.gauss_legendre_01(1)
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_inputs(a, d, alphas, weights, tag="_A")
+        synth.write_inputs(b, d, alphas, weights, tag="_B")
+        a.twoel_ao.tofile(os.path.join(d, "twoel_ao.bin"))
+        synth.twomoab_records(vab, a, b).astype("<f8").tofile(os.path.join(d, "TWOMOAB"))
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; {DRIVER} e2disp"], cwd=d, capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference driver failed\n{r.stdout[-3000:]}{r.stderr[-3000:]}")
+    res = dict(case=name, mode="e2disp", dimer=args, generator="gammcor_amd.synth.dimer")
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if t and t[0] == "MONOMER":
+            res[f"ndimx_{t[1]}"] = int(t[2])
+            res[f"ecasscf_{t[1]}"] = float(t[3])
+        elif t and t[0] == "E2DISP":
+            res["e2disp"] = float(t[1])
+    return res
+
 
 def parse(out):
     res = dict(walpha=[], alphas=[], weights=[], eig=[], twono=[])
@@ -142,6 +175,13 @@ def run_case(name):
 
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "small"
+    if which == "dimers" or which in DIMERS:
+        for name in (DIMERS if which == "dimers" else [which]):
+            res = run_dimer(name)
+            with open(os.path.join(HERE, f"ref_{name}.json"), "w") as f:
+                json.dump(res, f, indent=1)
+            print(name, {k: res.get(k) for k in ("ndimx_A", "ndimx_B", "e2disp")}, flush=True)
+        return
     names = SMALL if which == "small" else [n for n in CASES if n.startswith(which)] or [which]
     for name in names:
         res = run_case(name)

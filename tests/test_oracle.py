@@ -465,6 +465,71 @@ def test_ac0fofo_planted_mistakes_are_caught(monkeypatch):
     assert err() > 1e-6
 
 
+# ---------------------------------------------------------------- SAPT E2disp
+DIMERS = ["d12_cas44_e2disp", "d24_cas66_e2disp"]
+
+
+def _dimer(name):
+    import accas_oracle as O
+    from gammcor_amd import synth
+    fx = gold(name)
+    a, b, vab = synth.dimer(**fx["dimer"])
+    ra = O.sapt_response(O.system_dict(a), O.two_no1(a.twoel_ao, a.uaomo))
+    rb = O.sapt_response(O.system_dict(b), O.two_no1(b.twoel_ao, b.uaomo))
+    return fx, ra, rb, vab
+
+
+@pytest.mark.parametrize("name", DIMERS)
+def test_e2disp_matches_reference(name):
+    """The monomer responses (AB_CAS at alpha = 1, ERPASYMMXY) and the
+    coupled E2disp against the reference's own chain and e2disp
+    (sapt_pol.f90:1684) on the same seeded dimer."""
+    import accas_oracle as O
+    fx, ra, rb, vab = _dimer(name)
+    assert len(ra["indn"]) == fx["ndimx_A"] and len(rb["indn"]) == fx["ndimx_B"]
+    assert abs(ra["ecasscf"] - fx["ecasscf_A"]) < 1e-10 and abs(rb["ecasscf"] - fx["ecasscf_B"]) < 1e-10
+    e = O.e2disp(ra, rb, vab)
+    assert abs(e - fx["e2disp"]) < 1e-11 * max(1.0, abs(fx["e2disp"]))
+
+
+def test_e2disp_response_is_erpasymm0_y():
+    """(C_p - C_q)(EigY - EigX) of ERPASYMMXY is ERPASYMM0's Y: the
+    rescaling by 1/(C_p +- C_q) cancels."""
+    import accas_oracle as O
+    from gammcor_amd import synth
+    fx = gold("d12_cas44_e2disp")
+    a, _, _ = synth.dimer(**fx["dimer"])
+    sd = O.system_dict(a)
+    tw = O.two_no1(a.twoel_ao, a.uaomo)
+    r = O.sapt_response(sd, tw)
+    ap, am, _, _ = O.ab_cas(sd["nbasis"], sd["ninact"], sd["nact"], sd["occ"], sd["xone"], tw, sd["gamma_act"],
+                            sd["igem"], r["indn"], 1.0)
+    y0, _, eig = O.erpasymm0(np.asarray(ap), np.asarray(am))
+    assert np.max(np.abs(r["evec"] - y0)) < 1e-13
+    assert np.array_equal(r["eig"], eig)
+
+
+def test_e2disp_planted_mistakes_are_caught(monkeypatch):
+    """Without the (C_p + C_q)(C_r + C_s) factors, with EigY in place of the
+    response vectors, or with (pr|qs) in place of (pq|rs)_AB the result
+    leaves the reference's."""
+    import accas_oracle as O
+    fx, ra, rb, vab = _dimer("d12_cas44_e2disp")
+    ref = fx["e2disp"]
+    assert abs(O.e2disp(ra, rb, vab) - ref) < 1e-11
+    flat = dict(ra, cicoef=np.full_like(ra["cicoef"], 0.5))
+    assert abs(O.e2disp(flat, dict(rb, cicoef=np.full_like(rb["cicoef"], 0.5)), vab) - ref) > 1e-4
+    orig = O.erpasymmxy
+    monkeypatch.setattr(O, "erpasymmxy", lambda *a: (lambda y, x, e: (y, 0.0 * x, e))(*orig(*a)))
+    from gammcor_amd import synth
+    a, b, _ = synth.dimer(**fx["dimer"])
+    ra2 = O.sapt_response(O.system_dict(a), O.two_no1(a.twoel_ao, a.uaomo))
+    rb2 = O.sapt_response(O.system_dict(b), O.two_no1(b.twoel_ao, b.uaomo))
+    assert abs(O.e2disp(ra2, rb2, vab) - ref) > 1e-4
+    monkeypatch.setattr(O, "erpasymmxy", orig)
+    assert abs(O.e2disp(ra, rb, np.ascontiguousarray(vab.transpose(0, 2, 1, 3))) - ref) > 1e-4
+
+
 # ---------------------------------------------------------------- rdm2.dat
 def test_rdm2_dat_round_trip():
     from gammcor_amd import synth, host
