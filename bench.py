@@ -63,6 +63,68 @@ def load_traffic(config):
     return out
 
 
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_reference(cfg, nalpha, ndimx_full):
+    """The compiled reference (oracle/_ref/ref_driver: the reference's own
+    TwoNO1 and ACEInteg, Fortran + OpenBLAS) timed on a bounded sample: the
+    same kind of system at NBasis=80 (CAS(8,8), 4 inactive, one alpha
+    point), once on every host core and once on one core, scaled to the
+    workload by NBasis^5 (TwoNO1, misc.f:166) and NDimX^3 per alpha point
+    (ERPASYMM1's eigen solves and ACEneERPA's triple loop dominate
+    ACEInteg)."""
+    import subprocess
+    import tempfile
+    from gammcor_amd import synth
+    nb = 80
+    s = synth.make_system(nb, 4, cfg["nact"], cfg["nelact"], 3 * nb, cfg["sigma"], 11)
+    a, w = synth.gauss_legendre_01(1)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_inputs(s, d, a, w)
+        for thr in (cores, 1):
+            env = dict(os.environ, OMP_NUM_THREADS=str(thr), OPENBLAS_NUM_THREADS=str(thr))
+            r = subprocess.run(["bash", "-c", f"ulimit -s unlimited; {REF_DRIVER} grid"], cwd=d, env=env,
+                               capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError("reference driver failed: " + r.stderr[-500:])
+            t = {}
+            for line in r.stdout.splitlines():
+                p = line.split()
+                if p and p[0] == "TIME_TWONO1":
+                    t["twono1"] = float(p[1])
+                elif p and p[0] == "TIME_ACEINTEG":
+                    t["aceinteg"] = float(p[2])
+                elif p and p[0] == "NDIMX":
+                    t["ndimx"] = int(p[1])
+            out[thr] = t
+    nbf = cfg["nbasis"]
+
+    def solve_s(t):
+        return t["twono1"] * (nbf / nb) ** 5 + nalpha * t["aceinteg"] * (ndimx_full / t["ndimx"]) ** 3
+
+    full, one = solve_s(out[cores]), solve_s(out[1])
+    return dict(value=1.0 / full, unit="solves/s", cores=cores, kind="reference",
+                sample=f"compiled reference TwoNO1 + one ACEInteg at NBasis={nb} CAS({cfg['nelact']},{cfg['nact']}) "
+                       f"(NDimX={out[cores]['ndimx']}), scaled by NBasis^5 and NDimX^3 x {nalpha} alpha points",
+                cpu_model=cpu_model(), value_one_core=1.0 / one,
+                sample_seconds={str(k): v for k, v in out.items()},
+                seconds_per_solve_estimate=full)
+
+
 def cpu_baseline(s, cfg, nalpha, ao_host):
     """The oracle (CPU restatement, numpy/LAPACK + C) timed on a bounded
     sample of the same workload: the transform's per-column cost measured on
@@ -105,6 +167,72 @@ def cpu_baseline(s, cfg, nalpha, ao_host):
                 seconds_transform=t_transform, seconds_per_alpha=t_alpha)
 
 
+def bench_batch(args):
+    """The batching workload (64 systems, NBasis 40-140, 16 alpha points,
+    seed 3): systems transformed and resident in HBM before the timed region;
+    one step = ECorr of all 64 systems through the batching form
+    (gmc_acecorr_batch_dev), mixed precision; the FP64 batching form, its
+    HIP-graph replay and the systems solved one by one are timed beside it."""
+    import torch
+    from gammcor_amd import host, synth, _lib
+    dev = torch.device("cuda:0")
+    _lib.lib()
+    cfgs = synth.batch_configs()
+    nal = synth.BATCH["nalpha"]
+    a, w = host.GauLeg(0.0, 1.0, nal)
+    systems = []
+    for cfg in cfgs:
+        s = synth.make_system(with_integrals=False, **cfg)
+        ao = synth.ao_integrals_device(s.chol, dev)
+        ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(dev)
+        _lib.call("gmc_twono1_dev", ao.data_ptr(), ure.data_ptr(), s.nbasis, ao.numel(), None)
+        systems.append(host.DeviceSystem(ao, s.occ, s.xone, s.gamma_act, s.nbasis, s.ninact, s.nact, s.nele,
+                                         device=dev))
+    torch.cuda.synchronize()
+    nsys = len(systems)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, r
+
+    t_mix, r_mix = timed(lambda: host.ACECORR_BATCH(systems, a, w, prec=host.GMC_MIXED), args.steps, args.warmup)
+    t_64, r_64 = timed(lambda: host.ACECORR_BATCH(systems, a, w, prec=host.GMC_FP64), 1, 1)
+    t_gr, r_gr = timed(lambda: host.ACECORR_BATCH(systems, a, w, prec=host.GMC_FP64, graph=True), 1, 1)
+    t_one, e_one = timed(lambda: np.array([host.ACECORR(d, a, w, prec=host.GMC_FP64)["ecorr"] for d in systems]), 1, 1)
+    grp, npad = host.batch_plan([d.ndimx for d in systems], nal)
+    line = {
+        "metric": "AC-CAS solves/s, batching workload: 64 systems (NBasis 40-140) x 16 alpha points",
+        "value": nsys / t_mix,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mix,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 eigen solves + f64 (Ogita-Aishima refinement, assembly, quadrature)",
+        "data": "synthetic (seeded Cholesky-factor AO integrals, QR orbitals, random-CI RDMs)",
+        "config": {"workload": "batch64: 64 systems NBasis uniform in [40,140] (seed 3), CAS(4,4)/CAS(6,6) alternating, "
+                               "16 Gauss-Legendre alpha points, all in the batching form",
+                   "groups": int(grp.max() + 1), "padded_orders": sorted(set(npad.tolist()), reverse=True)},
+        "fp64_batch_solves_per_s": nsys / t_64,
+        "fp64_batch_graph_solves_per_s": nsys / t_gr,
+        "fp64_one_by_one_solves_per_s": nsys / t_one,
+        "max_abs_decorr_mixed_vs_fp64": float(np.max(np.abs(r_mix["ecorr"] - r_64["ecorr"]))),
+        "max_abs_decorr_batch_vs_one_by_one": float(np.max(np.abs(r_64["ecorr"] - e_one))),
+        "max_abs_decorr_graph_vs_eager": float(np.max(np.abs(r_gr["ecorr"] - r_64["ecorr"]))),
+        "cost_model": None if host.batch_cost() is None else host.batch_cost().tolist(),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +255,8 @@ def main():
               "multi-GPU sharding (alpha points per GPU, one 8-byte all-reduce) is not run.", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "batch64":
+        return bench_batch(args)
     import torch
     from gammcor_amd import host, synth, _lib
     dev = torch.device("cuda:0")
@@ -283,7 +413,10 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(s, cfg, nalpha, ao.cpu().numpy())
+        if os.path.exists(REF_DRIVER):
+            line["cpu_baseline"] = cpu_baseline_reference(cfg, nalpha, dsys.ndimx)
+        else:
+            line["cpu_baseline"] = cpu_baseline(s, cfg, nalpha, ao.cpu().numpy())
     print(json.dumps(line), flush=True)
 
 

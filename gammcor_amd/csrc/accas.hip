@@ -15,6 +15,7 @@
 #include <hip/hip_fp16.h>
 
 #include "accas.hpp"
+#include "chol.hpp"
 #include "oa_policy.hpp"
 #include "gemm.hpp"
 #include "prof.hpp"
@@ -487,6 +488,104 @@ __global__ __launch_bounds__(256) void form_kernel(const double* p0, const doubl
     }
 }
 
+// The batching form: A+(alpha) and A-(alpha) of system k of a batch,
+// padded to the batch order npad.  Within the system's order n the matrices
+// are those of form_kernel; beyond it they are zero.  The padded modes stay
+// decoupled through every step: their reflector tails are zero (tau = 0),
+// their divide & conquer weights z are exactly zero so they deflate before
+// any close-pair rotation, and with eigenvalue 0 they leave the deflation
+// tolerance (8 eps max(|d|, |z|) of a merge) as it is without them - a pad
+// above the spectrum would raise it; S, M and Y are zero on them, and the
+// integrand reads only the system's own block.  One workgroup per 64 x 64
+// tile on or below the diagonal and system (blockIdx.y), writing the tile of
+// every alpha point.
+template <class T>
+__global__ __launch_bounds__(256) void form_pad_kernel(const BatchSys* sys, const double* alphas, int nalpha,
+                                                       T* Ap, T* Am, int npad)
+{
+    __shared__ double s0[64][65], s1[64][65], tmp[64][65];
+    int ti, tj;
+    lower_tile(blockIdx.x, ti, tj);
+    const BatchSys sy = sys[blockIdx.y];
+    const int n = sy.n;
+    const int i0 = ti * 64, j0 = tj * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    auto load_sym = [&](const double* X, double (*dst)[65]) {
+        for (int y = ty; y < 64; y += 4) {
+            const int i = i0 + tx, j = j0 + y;
+            dst[y][tx] = (i < n && j < n) ? X[i + (size_t)j * n] : 0.0;
+            const int i2 = j0 + tx, j2 = i0 + y;
+            tmp[y][tx] = (i2 < n && j2 < n) ? X[i2 + (size_t)j2 * n] : 0.0;
+        }
+        __syncthreads();
+        for (int y = ty; y < 64; y += 4) dst[y][tx] = 0.5 * (dst[y][tx] + tmp[tx][y]);
+        __syncthreads();
+    };
+    for (int which = 0; which < 2; which++) {
+        load_sym(which ? sy.m0 : sy.p0, s0);
+        load_sym(which ? sy.m1 : sy.p1, s1);
+        T* out = which ? Am : Ap;
+        for (int a = 0; a < nalpha; a++) {
+            const double al = alphas[a];
+            T* o = out + ((size_t)blockIdx.y * nalpha + a) * npad * npad;
+            for (int y = ty; y < 64; y += 4) {
+                const int i = i0 + tx, j = j0 + y;
+                if (i < npad && j < npad) {
+                    const double v = (i < n && j < n) ? (1.0 - al) * s0[y][tx] + al * s1[y][tx] : 0.0;
+                    o[i + (size_t)j * npad] = (T)v;
+                }
+                const int i2 = j0 + tx, j2 = i0 + y;
+                if (ti != tj && i2 < npad && j2 < npad) {
+                    const double v = (i2 < n && j2 < n) ? (1.0 - al) * s0[tx][y] + al * s1[tx][y] : 0.0;
+                    o[i2 + (size_t)j2 * npad] = (T)v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// W of the batching form: problem b = k nalpha + a, system k's weights over
+// its own order n (P is padded to npad); diag + 2 x strictly lower.
+template <class T>
+__global__ __launch_bounds__(256) void dot_batch_kernel(const BatchSys* sys, int nalpha, const T* P, int npad,
+                                                        double* partial)
+{
+    __shared__ double red[4];
+    const int b = blockIdx.y;
+    const BatchSys sy = sys[b / nalpha];
+    const int n = sy.n;
+    const T* pm = P + (size_t)b * npad * npad;
+    const int64_t total = (int64_t)n * n;
+    double acc = 0.0;
+    for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int i = (int)(idx % n), j = (int)(idx / n);
+        const double g = sy.gw[idx], pv = (double)pm[i + (size_t)j * npad];
+        if (i > j) acc += 2.0 * g * pv;
+        else if (i == j) acc += g * pv;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void sum_partials_batch_kernel(const double* partial, int np, const BatchSys* sys, int nalpha, double* W,
+                                          int nb)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double s = 0.0;
+    for (int i = 0; i < np; i++) s += partial[(size_t)b * np + i];
+    W[b] = s + sys[b / nalpha].wdiag;
+}
+
+// x[0..n) = 0 (a kernel, not a memset: the solve sequence may be captured)
+__global__ void zero_int_kernel(int* x, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
 // V' = V |lambda|^(1/4) column-wise; count negative eigenvalues of A+.
 template <class T>
 __global__ void sqrt_scale_kernel(T* V, const T* lam, int n, int* nneg)
@@ -590,7 +689,7 @@ int erpa_plan(int n, int batch, ErpaPlan<T>* p)
 template <class T>
 int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
                  const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out, double* S_out,
-                 double* Am_out)
+                 double* Am_out, const ErpaBatchTables* bt)
 {
     GMC_CHECK(nalpha >= 1 && nalpha <= p.batch, "ERPA: %d alpha points for a plan of %d", nalpha, p.batch);
     GMC_CHECK(p.eig && p.eig->bound && p.eig->batch == p.batch && p.eig->n == p.n, "ERPA: eigensolver plan unbound");
@@ -607,41 +706,51 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     const int npart = 1024;
     double* partial = ar.take<double>((size_t)p.batch * npart);
     GMC_CHECK(ar.fits(), "ERPA: scratch layout overflow");
-    GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
-    GMC_HIP(hipMemsetAsync(out.nneg, 0, nalpha * sizeof(int), st));
+    const int nprob = bt ? bt->nsys * nalpha : nalpha;   // problems of this call
+    GMC_CHECK(nprob <= p.batch, "ERPA: %d problems for a plan of %d", nprob, p.batch);
+    hipLaunchKernelGGL(zero_int_kernel, dim3((nprob + 255) / 256), dim3(256), 0, st, out.nneg, nprob);
 
-    const dim3 gsq((n + 255) / 256, n, B), blk(256);
+    const dim3 blk(256);
     // A+(alpha) -> E, A-(alpha) -> Tb
     const int nt64f = (n + 63) / 64;
-    hipLaunchKernelGGL(form_kernel<T>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1, d_al, B,
-                       E, Tb, n);
+    if (bt) {
+        // the batching form: alphas are already on the device
+        hipLaunchKernelGGL(form_pad_kernel<T>, dim3(nt64f * (nt64f + 1) / 2, bt->nsys), blk, 0, st, bt->sys, alphas,
+                           nalpha, E, Tb, n);
+    } else {
+        GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(form_kernel<T>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1,
+                           d_al, B, E, Tb, n);
+    }
     GMC_HIP(hipGetLastError());
+    const int Bp = nprob;   // the solve below runs over every problem
+    const dim3 gsq((n + 255) / 256, n, Bp);
     if (Am_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: A- output is available from the FP64 solve");
-        GMC_HIP(hipMemcpyAsync(Am_out, Tb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+        GMC_HIP(hipMemcpyAsync(Am_out, Tb, Bp * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
     EigPlan<T>& ep = *p.eig;
-    GMC_CHECK(ep.batch == B, "ERPA: plan batch %d != %d alpha points", ep.batch, B);
+    GMC_CHECK(ep.batch == Bp, "ERPA: plan batch %d != %d problems", ep.batch, Bp);
     GMC_TRY(syevd_batched<T>(st, ep, E, lam));
     hipLaunchKernelGGL(sqrt_scale_kernel<T>, gsq, blk, 0, st, E, lam, n, out.nneg);
     GMC_HIP(hipGetLastError());
     // S = V|L|^(1/2)V^T
     // S is symmetric: its lower triangle, mirrored
-    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, Bp, 1};
     GMC_TRY(gemm_big<T>(st, gS, T(1), E, E, T(0), Sb));
     const int nt64 = (n + 63) / 64;
-    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, B), blk, 0, st, Sb, n);
+    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, Bp), blk, 0, st, Sb, n);
     if (S_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: A+^(1/2) output is available from the FP64 solve");
-        GMC_HIP(hipMemcpyAsync(S_out, Sb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+        GMC_HIP(hipMemcpyAsync(S_out, Sb, Bp * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
     // M = S A- S
-    GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
+    GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, Bp, 0};
     GMC_TRY(gemm_big<T>(st, gN, T(1), Tb, Sb, T(0), Mb));
     GemmShape gNl = gN;
     gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
     GMC_TRY(gemm_big<T>(st, gNl, T(1), Sb, Mb, T(0), Tb));
-    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, B), blk, 0, st, Tb, n);
+    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(nt64, nt64, Bp), blk, 0, st, Tb, n);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<T>(st, ep, Tb, om2));
     // Y = S Z
@@ -649,16 +758,22 @@ int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double*
     hipLaunchKernelGGL(ycols_kernel<T>, gsq, blk, 0, st, Mb, E, om2, out.eig, n);
     GMC_HIP(hipGetLastError());
     // P = Yf Yf^T (lower) and W (skipped without integrand weights)
-    if (Gw) {
-        GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    if (Gw || bt) {
+        GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, Bp, 1};
         GMC_TRY(gemm_big<T>(st, gP, T(1), E, E, T(0), Sb));
-        hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
-        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+        if (bt) {
+            hipLaunchKernelGGL(dot_batch_kernel<T>, dim3(npart, Bp), blk, 0, st, bt->sys, nalpha, Sb, n, partial);
+            hipLaunchKernelGGL(sum_partials_batch_kernel, dim3((Bp + 63) / 64), dim3(64), 0, st, partial, npart,
+                               bt->sys, nalpha, out.W, Bp);
+        } else {
+            hipLaunchKernelGGL(dot_lower_kernel<T>, dim3(npart, B), blk, 0, st, Gw, Sb, n, partial);
+            hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+        }
         GMC_HIP(hipGetLastError());
     }
     if (Y_out) {
         GMC_CHECK(sizeof(T) == 8, "ERPA: Y output is available from the FP64 solve");
-        GMC_HIP(hipMemcpyAsync(Y_out, Mb, B * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
+        GMC_HIP(hipMemcpyAsync(Y_out, Mb, Bp * nn * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
     return 0;
 }
@@ -990,19 +1105,131 @@ size_t erpa_mixed_bytes(int n, int batch)
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
                         B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4,
-                        B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8});
+                        B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8, potrf_work_bytes(n, batch), B * 4});
 }
 
-// Mixed ERPA: eigen solves of A+ and of M = S A- S in FP32, each refined in
-// FP64 by n_min to n_max Ogita-Aishima iterations (n_min >= 1), problems that
-// do not converge solved again in FP64 when `resolve` is given; S, M, Y and
-// the integrand in FP64.
+// ------------------------------------------------------------ Cholesky route
+// The reference's S = |A+|^(1/2) (interpa.f:1693-1716) enters only through
+// M = S A- S and Y = S Z.  Any factor L of |A+| = L L^T serves as well: L =
+// S Q with Q orthogonal, L^T A- L = Q^T M Q has the eigenvectors Q^T Z, and
+// L Q^T Z = S Z.  |A+| = A+ - 2 sum_{lambda < 0} lambda x x^T needs only the
+// few eigenpairs of A+ below zero to FP64 accuracy: the K smallest columns of
+// the FP32 eigenvector matrix (every negative eigenvalue and at least 8 more)
+// are refined by Ogita-Aishima steps restricted to those columns, the FP32
+// basis and eigenvalues standing in for the rest.  Each step costs O(n^2 K)
+// and gains about the FP32 basis' accuracy, so two or three steps reach the
+// FP64 rounding floor, where the full refinement of A+ costs three O(n^3)
+// steps.
+
+// Negative FP32 eigenvalues per problem (ascending order).
+__global__ void count_neg_kernel(const float* w, int n, int B, int* cnt)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int c = 0;
+    while (c < n && w[(size_t)b * n + c] < 0.f) c++;
+    cnt[b] = c;
+}
+
+// lambda_j = s_jj / g_jj of the refined columns (S = X^T A X_J, G = X^T X_J,
+// n x K, ld n); delta_b = 64 eps max |lambda| (the cluster threshold).
+__global__ void oac_lam_kernel(const double* S, const double* G, const float* w32, int n, int K, double* lamJ,
+                               double* delta)
+{
+    const int b = blockIdx.x;
+    const size_t nn = (size_t)n * n;
+    double mx = fmax(fabs((double)w32[(size_t)b * n]), fabs((double)w32[(size_t)b * n + n - 1]));
+    for (int j = threadIdx.x; j < K; j += 64) {
+        const size_t o = (size_t)b * nn + (size_t)j * n + j;
+        const double l = S[o] / G[o];
+        lamJ[(size_t)b * K + j] = l;
+        mx = fmax(mx, fabs(l));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (threadIdx.x == 0) delta[b] = 64.0 * 2.220446049250313e-16 * mx;
+}
+
+// E_ij of the restricted step (i < n, j < K): (s_ij - l_j g_ij) / (l_j - l_i)
+// for separated pairs, -g_ij / 2 for close ones (the same rule as the full
+// step), (1 - g_jj) / 2 on the diagonal; l_i of an unrefined column is its
+// FP32 eigenvalue.  emax[b]: the largest |E_ij| off the diagonal, as float
+// bits (+inf for a non-finite element).
+__global__ __launch_bounds__(256) void oac_e_kernel(const double* S, const double* G, const double* lamJ,
+                                                    const float* w32, const double* delta, int n, int K, double* E,
+                                                    unsigned* emax)
+{
+    const int b = blockIdx.z, j = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const size_t o = (size_t)b * n * n + (size_t)j * n + i;
+    float ea = 0.f;
+    if (i < n) {
+        const double s = S[o], g = G[o];
+        double e;
+        if (i == j) {
+            e = 0.5 * (1.0 - g);
+        } else {
+            const double lj = lamJ[(size_t)b * K + j];
+            const double li = i < K ? lamJ[(size_t)b * K + i] : (double)w32[(size_t)b * n + i];
+            const double gap = lj - li;
+            e = (s - lj * g) / gap;
+            if (!(fabs(gap) > delta[b]) || !(fabs(e) < 0.125)) e = -0.5 * g;
+            ea = isfinite(e) ? (float)fabs(e) : INFINITY;
+        }
+        E[o] = e;
+    }
+    for (int off = 32; off > 0; off >>= 1) ea = fmaxf(ea, __shfl_xor(ea, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&emax[b], __float_as_uint(ea));
+}
+
+// X_J += P over the first K columns of each problem.
+__global__ void oac_add_kernel(double* X, const double* P, int n, int K)
+{
+    const size_t o = (size_t)blockIdx.y * n * n;
+    const int64_t cnt = (int64_t)n * K;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+        X[o + i] += P[o + i];
+}
+
+// T = X_J diag(-2 min(lambda_j, 0)) (n x K, ld n); nneg counts lambda_j < 0.
+__global__ void oac_reflect_kernel(const double* X, const double* lamJ, int n, int K, double* T, int* nneg)
+{
+    const int b = blockIdx.z, j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double l = lamJ[(size_t)b * K + j];
+    const double d = l < 0.0 ? -2.0 * l : 0.0;
+    const size_t o = (size_t)b * n * n + (size_t)j * n + i;
+    if (i < n) T[o] = d * X[o];
+    if (i == 0 && l < 0.0) atomicAdd(&nneg[b], 1);
+}
+
+// The batching form: a padded mode has a zero row and column in |A+|; a unit
+// diagonal there keeps L a factor of the system's own block (and the pad
+// block of M = L^T A- L zero).
+__global__ void pad_diag_one_kernel(const BatchSys* sys, int nalpha, double* A, int npad)
+{
+    const int k = blockIdx.y;
+    const int n = sys[k].n;
+    for (int a = 0; a < nalpha; a++) {
+        double* Ab = A + ((size_t)k * nalpha + a) * npad * npad;
+        for (int i = n + blockIdx.x * blockDim.x + threadIdx.x; i < npad; i += gridDim.x * blockDim.x)
+            Ab[(size_t)i * npad + i] = 1.0;
+    }
+}
+
+// Mixed ERPA: eigen solves of A+ and of M in FP32.  A+ gives the factor F
+// of |A+| = F F^T: its Cholesky factor (the Cholesky route above), or S =
+// |A+|^(1/2) from the full refinement when that route fails; M = F^T A- F.
+// M's eigenpairs are refined in FP64 by n_min to n_max Ogita-Aishima
+// iterations (n_min >= 1), problems that do not converge solved again in
+// FP64 when `resolve` is given; F, M, Y and the integrand in FP64.
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from)
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from,
+                       const ErpaBatchTables* bt)
 {
-    GMC_CHECK(ep.bound && ep.batch == nalpha && ep.n == n, "ERPA mixed: eigensolver plan mismatch");
-    const int B = nalpha;
+    // B problems: the alpha points, or (batching form) every system at every alpha point
+    const int B = bt ? bt->nsys * nalpha : nalpha;
+    GMC_CHECK(ep.bound && ep.batch == B && ep.n == n, "ERPA mixed: eigensolver plan mismatch");
     const size_t nn = (size_t)n * n;
     Arena ar{static_cast<char*>(scratch), scratch_bytes};
     double* Ap = ar.take<double>(B * nn);
@@ -1031,6 +1258,8 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     // (diagnostics: forces the over-the-cap path)
     if (const char* e = getenv("GMC_OA_MAXPAIRS")) op.cap = (unsigned)std::min(std::max(0, atoi(e)), OA_MAXPAIRS);
     double* wre = ar.take<double>(B * n);
+    char* pwork = ar.take<char>(potrf_work_bytes(n, B));
+    int* info = ar.take<int>(B);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
     auto report = [&](const char* what, int it) -> int {
@@ -1096,35 +1325,111 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         return 0;
     };
     std::vector<int> left;
-    GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
-    GMC_HIP(hipMemsetAsync(out.nneg, 0, nalpha * sizeof(int), st));
+    hipLaunchKernelGGL(zero_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st, out.nneg, B);
     const dim3 gsq((n + 255) / 256, n, B), blk(256);
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
 
     const int nt64f = (n + 63) / 64;
-    hipLaunchKernelGGL(form_kernel<double>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0, ab.m1,
-                       d_al, B, Ap, Am, n);
+    if (bt) {
+        // the batching form: alphas are already on the device
+        hipLaunchKernelGGL(form_pad_kernel<double>, dim3(nt64f * (nt64f + 1) / 2, bt->nsys), blk, 0, st, bt->sys,
+                           alphas, nalpha, Ap, Am, n);
+    } else {
+        GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(form_kernel<double>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0,
+                           ab.m1, d_al, B, Ap, Am, n);
+    }
     // eig(A+) in FP32, refined in FP64
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
-    GMC_TRY(refine("A+", Ap, lam, AX, left));
-    GMC_TRY(resolve_left("A+", left, Ap, lam));
-    // S = V|L|^(1/2)V^T (into AX), M = S A- S (into Ap)
-    hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
-    GMC_HIP(hipGetLastError());
-    // S is symmetric: its lower triangle, mirrored
-    GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-    GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
     const int nt64 = (n + 63) / 64;
-    hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, AX, n);
+    // The Cholesky route: L L^T = |A+| into AX.  ok stays false when the
+    // restricted refinement does not reach OAC_TOL in OAC_MAX_STEPS steps or a
+    // pivot is not positive; the eigen route below then takes over from the
+    // (partly refined) X.
+    auto chol_route = [&](bool& ok) -> int {
+        ok = false;
+        ProfScope ps(st, "mixed_chol_route", 0.0);
+        hipLaunchKernelGGL(count_neg_kernel, dim3((B + 255) / 256), dim3(256), 0, st, w32, n, B, info);
+        std::vector<int> hc(B);
+        GMC_HIP(hipMemcpyAsync(hc.data(), info, B * sizeof(int), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        const int maxneg = *std::max_element(hc.begin(), hc.end());
+        const int K = std::min(n, (maxneg + 8 + 15) / 16 * 16);
+        const int64_t snn = (int64_t)nn;
+        GemmShape gAX{n, K, n, false, false, n, n, n, snn, snn, snn, B, 0};
+        GemmShape gTX{n, K, n, true, false, n, n, n, snn, snn, snn, B, 0};
+        const dim3 gk((n + 255) / 256, K, B);
+        const unsigned gadd = (unsigned)std::min<int64_t>(4096, ((int64_t)n * K + 255) / 256);
+        std::vector<unsigned> em(B);
+        bool conv = false;
+        for (int it = 0; it < OAC_MAX_STEPS && !conv; it++) {
+            GMC_TRY(gemm_big<double>(st, gAX, 1.0, Ap, X, 0.0, AX));   // A X_J
+            GMC_TRY(gemm_big<double>(st, gTX, 1.0, X, AX, 0.0, Sb));   // X^T A X_J
+            GMC_TRY(gemm_big<double>(st, gTX, 1.0, X, X, 0.0, Gb));    // X^T X_J
+            hipLaunchKernelGGL(oac_lam_kernel, dim3(B), dim3(64), 0, st, Sb, Gb, w32, n, K, lam, delta);
+            hipLaunchKernelGGL(zero_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st,
+                               reinterpret_cast<int*>(op.emax), B);
+            hipLaunchKernelGGL(oac_e_kernel, gk, blk, 0, st, Sb, Gb, lam, w32, delta, n, K, Xn, op.emax);
+            GMC_TRY(gemm_big<double>(st, gAX, 1.0, X, Xn, 0.0, AX));   // X E_J
+            hipLaunchKernelGGL(oac_add_kernel, dim3(gadd, B), blk, 0, st, X, AX, n, K);
+            GMC_HIP(hipGetLastError());
+            GMC_HIP(hipMemcpyAsync(em.data(), op.emax, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            GMC_HIP(hipStreamSynchronize(st));
+            float mx = 0.f;
+            for (unsigned u : em) {
+                float f;
+                std::memcpy(&f, &u, 4);
+                mx = std::max(mx, f);   // +inf stays +inf
+            }
+            if (stats) fprintf(stderr, "[gmc] Cholesky route: restricted step %d on %d columns, max |E| %.3e\n", it, K, mx);
+            conv = mx < OAC_TOL;
+        }
+        if (!conv) return 0;
+        // |A+| (lower triangle) into AX, then its Cholesky factor
+        GMC_HIP(hipMemcpyAsync(AX, Ap, (size_t)B * nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(oac_reflect_kernel, gk, blk, 0, st, X, lam, n, K, Sb, out.nneg);
+        GemmShape gR{n, n, K, false, true, n, n, n, snn, snn, snn, B, 1};
+        GMC_TRY(gemm_big<double>(st, gR, 1.0, Sb, X, 1.0, AX));
+        if (bt) hipLaunchKernelGGL(pad_diag_one_kernel, dim3(std::max(1, (n + 255) / 256), bt->nsys), blk, 0, st,
+                                   bt->sys, nalpha, AX, n);
+        GMC_HIP(hipGetLastError());
+        GMC_TRY(potrf_batched(st, n, B, AX, pwork, info));
+        GMC_HIP(hipMemcpyAsync(hc.data(), info, B * sizeof(int), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        int bad = 0;
+        for (int v : hc) bad += v != 0;
+        if (bad) {
+            if (stats) fprintf(stderr, "[gmc] Cholesky route: %d of %d factorisations failed\n", bad, B);
+            hipLaunchKernelGGL(zero_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st, out.nneg, B);
+            return 0;
+        }
+        ok = true;
+        return 0;
+    };
+    bool chol_ok = false;
+    const char* route = getenv("GMC_MIXED_ROUTE");
+    if (!(route && strcmp(route, "eigen") == 0)) GMC_TRY(chol_route(chol_ok));
+    if (!chol_ok) {
+        // the eigen route of the reference: A+ refined in full, S = V|L|^(1/2)V^T (into AX)
+        ProfScope ps(st, "mixed_eigen_route", 0.0);
+        GMC_TRY(refine("A+", Ap, lam, AX, left));
+        GMC_TRY(resolve_left("A+", left, Ap, lam));
+        hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
+        GMC_HIP(hipGetLastError());
+        // S is symmetric: its lower triangle, mirrored
+        GemmShape gS{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+        GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
+        hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, AX, n);
+    }
+    // M = F^T A- F (into Ap) for the factor F in AX (L, or S = S^T)
     GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
     GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
-    GemmShape gNl = gN;
-    gNl.uplo = 1;   // M is symmetric: its lower triangle, mirrored
-    GMC_TRY(gemm_big<double>(st, gNl, 1.0, AX, Sb, 0.0, Ap));
+    GemmShape gTl{n, n, n, true, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
+    GMC_TRY(gemm_big<double>(st, gTl, 1.0, AX, Sb, 0.0, Ap));   // M is symmetric: its lower triangle, mirrored
     hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, Ap, n);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
@@ -1134,13 +1439,19 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     GMC_HIP(hipGetLastError());
     GMC_TRY(refine("M", Ap, om2, Am, left));
     GMC_TRY(resolve_left("M", left, Ap, om2));
-    // Y = S Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
+    // Y = F Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
     GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
     hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);
     GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
     GMC_TRY(gemm_big<double>(st, gP, 1.0, Gb, Gb, 0.0, Am));
-    hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, B), blk, 0, st, Gw, Am, n, partial);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+    if (bt) {
+        hipLaunchKernelGGL(dot_batch_kernel<double>, dim3(npart, B), blk, 0, st, bt->sys, nalpha, Am, n, partial);
+        hipLaunchKernelGGL(sum_partials_batch_kernel, dim3((B + 63) / 64), dim3(64), 0, st, partial, npart, bt->sys,
+                           nalpha, out.W, B);
+    } else {
+        hipLaunchKernelGGL(dot_lower_kernel<double>, dim3(npart, B), blk, 0, st, Gw, Am, n, partial);
+        hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, npart, wdiag, out.W, B);
+    }
     GMC_HIP(hipGetLastError());
     return 0;
 }
@@ -1148,9 +1459,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
 template int erpa_plan<double>(int, int, ErpaPlan<double>*);
 template int erpa_plan<float>(int, int, ErpaPlan<float>*);
 template int erpa_batched<double>(hipStream_t, ErpaPlan<double>&, const AbPair&, const double*, int, const double*,
-                                  const double*, void*, ErpaOut&, double*, double*, double*);
+                                  const double*, void*, ErpaOut&, double*, double*, double*, const ErpaBatchTables*);
 template int erpa_batched<float>(hipStream_t, ErpaPlan<float>&, const AbPair&, const double*, int, const double*,
-                                 const double*, void*, ErpaOut&, double*, double*, double*);
+                                 const double*, void*, ErpaOut&, double*, double*, double*, const ErpaBatchTables*);
 
 // ------------------------------------------------------------ integrand from given Y
 // Columns of Y whose Eig lies in (1e-3, 1e8) (acfd.f:4976, 5013), others zero.

@@ -55,16 +55,35 @@ struct ErpaPlan {
     size_t ws_bytes = 0;         // scratch beyond the eigensolver's
 };
 
+// Several systems of uneven order solved in one batch (the batching form):
+// each system's A(0), A(1) and integrand weights and its order n (its
+// problems are padded with zeros to the batch order).  Problem
+// b = k * nalpha + a of a batch is system k at alpha point a.
+struct BatchSys {
+    const double *p0, *m0, *p1, *m1;   // A(0)+-, A(1)+- (n x n)
+    const double* gw;                   // integrand weights (n x n)
+    double wdiag;                       // alpha-independent diagonal term of W
+    int n;
+};
+struct ErpaBatchTables {
+    const BatchSys* sys = nullptr;      // device, one per system of the batch
+    int nsys = 0;
+};
+
 template <class T>
 int erpa_plan(int n, int batch, ErpaPlan<T>* p);
 
 // alphas: host array of nalpha <= plan batch values.
 // Optional outputs (FP64 only): Y_out the normalised Y = A+^(1/2) Z columns,
 // S_out = A+^(1/2), Am_out the symmetrised A-.  Gw == nullptr skips the integrand.
+// With bt (the batching form): the problems are bt->nsys systems x nalpha
+// alpha points, each system's matrices padded to the plan's order (ab, Gw and
+// wdiag unused; alphas then on the device); the padded modes are decoupled
+// and carry no integrand weight.
 template <class T>
 int erpa_batched(hipStream_t st, ErpaPlan<T>& p, const AbPair& ab, const double* alphas, int nalpha,
                  const double* Gw, const double* wdiag, void* scratch, ErpaOut& out, double* Y_out = nullptr,
-                 double* S_out = nullptr, double* Am_out = nullptr);
+                 double* S_out = nullptr, double* Am_out = nullptr, const ErpaBatchTables* bt = nullptr);
 
 // W = sum Gw o (Yf Yf^T) + wdiag for a given normalised Y and Eig (ACEneERPA,
 // acfd.f:4968-5043), Yf the columns with 1e-3 < Eig < 1e8.
@@ -92,6 +111,13 @@ constexpr float OA_GIVEUP = 5e-2f;
 // leaves.
 constexpr float OA_HALF = 5e-3f;
 
+// The Cholesky route's restricted refinement of the eigenpairs of A+ below
+// zero: at most OAC_MAX_STEPS steps to a largest correction under OAC_TOL
+// (the FP64 rounding floor of a correction is about 1e-13; each step from the
+// FP32 start gains about four digits).
+constexpr int OAC_MAX_STEPS = 6;
+constexpr float OAC_TOL = 1e-11f;
+
 // FP64 eigensolve of nb problems side by side (stride n*n, overwritten by
 // their eigenvectors; w gets the eigenvalues, ascending).
 using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
@@ -107,6 +133,7 @@ using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
 size_t erpa_mixed_bytes(int n, int batch);
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
-                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from);
+                       int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from,
+                       const ErpaBatchTables* bt = nullptr);
 
 }  // namespace gmc

@@ -20,6 +20,11 @@ struct AcRun {
 int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
                const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
                AcRun& r);
+// The same in the caller's device memory (at least ac_prepare_bytes).
+size_t ac_prepare_bytes(int N, int nI, int nA, int ndimx);
+int ac_prepare_at(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
+                  const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
+                  void* mem, size_t mem_bytes, AcRun& r);
 
 // The alpha points in batches of `cap` through the ERPA route of `prec`;
 // walpha and nneg_host are host arrays, eig_dev (optional) a device array.

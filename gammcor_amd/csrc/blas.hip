@@ -69,11 +69,19 @@ static int tri_parts()
     return parts;
 }
 
+static thread_local bool t_own_gemm = false;
+bool set_own_gemm(bool on)
+{
+    const bool was = t_own_gemm;
+    t_own_gemm = on;
+    return was;
+}
+
 template <class T>
 int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C)
 {
     static const bool own = getenv("GMC_OWN_GEMM") != nullptr;   // diagnostic: the library's own kernel
-    if (own) return gemm<T>(s, g, alpha, A, B, beta, C);
+    if (own || t_own_gemm) return gemm<T>(s, g, alpha, A, B, beta, C);
     constexpr int TRI_MIN = 1024;   // below this a single full product is as fast
     if (g.uplo != 1 || g.m != g.n || g.m < TRI_MIN) return gemm_one<T>(s, g, alpha, A, B, beta, C);
     const int P = tri_parts();

@@ -107,12 +107,33 @@ __global__ void expand_rdm2_kernel(const double* packed, double* gam, int nA)
 
 namespace gmc {
 
+size_t ac_prepare_bytes(int N, int nI, int nA, int ndimx)
+{
+    AcSystem sys;
+    sys.N = N;
+    sys.nI = nI;
+    sys.nA = nA;
+    sys.ndimx = ndimx;
+    const size_t nd2 = (size_t)ndimx * ndimx, na4 = (size_t)nA * nA * nA * nA;
+    return carve_bytes({nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, (size_t)N * 8, (na4 + 1) * 8, 8,
+                        ab_scratch_bytes(sys)});
+}
+
 // Carve A(0), A(1), weights and the assembly scratch from st.acws.
 int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
                       const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
                       AcRun& r)
 {
     State& st = state();
+    const size_t need = ac_prepare_bytes(N, nI, nA, ndimx);
+    GMC_TRY(st.acws.reserve(need));
+    return ac_prepare_at(s, N, nI, nA, ndimx, twono, occ, igem, xone, ure, rdm2act, indn, st.acws.ptr, need, r);
+}
+
+int ac_prepare_at(hipStream_t s, int N, int nI, int nA, int ndimx, const double* twono, const double* occ,
+                  const int* igem, const double* xone, const double* ure, const double* rdm2act, const int* indn,
+                  void* mem, size_t mem_bytes, AcRun& r)
+{
     const size_t nd2 = (size_t)ndimx * ndimx;
     AcSystem sys;
     sys.N = N;
@@ -126,10 +147,8 @@ int ac_prepare(hipStream_t s, int N, int nI, int nA, int ndimx, const double* tw
     sys.ure = ure;
     sys.indn = indn;
     const size_t na4 = (size_t)nA * nA * nA * nA;
-    const size_t need = carve_bytes({nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, nd2 * 8, (size_t)N * 8, (na4 + 1) * 8, 8,
-                                     ab_scratch_bytes(sys)});
-    GMC_TRY(st.acws.reserve(need));
-    Arena ar{static_cast<char*>(st.acws.ptr), need};
+    GMC_CHECK(mem_bytes >= ac_prepare_bytes(N, nI, nA, ndimx), "ACECORR: workspace too small");
+    Arena ar{static_cast<char*>(mem), mem_bytes};
     r.ab.p0 = ar.take<double>(nd2);
     r.ab.m0 = ar.take<double>(nd2);
     r.ab.p1 = ar.take<double>(nd2);

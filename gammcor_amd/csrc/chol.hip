@@ -1,0 +1,152 @@
+// Batched FP64 Cholesky factorisation A = L L^T (right-looking, panels of
+// CHOL_NB columns).  The mixed ERPA path factors |A+| with it (the reference
+// takes |A+|^(1/2) from an eigendecomposition instead, interpa.f:1693-1716).
+//
+// Per panel k: one workgroup per problem factors the diagonal block in LDS
+// and inverts its factor (forward substitution, one column per thread); the
+// panel below becomes L21 = A21 L11^-T (a product with the small inverse)
+// and the trailing lower triangle takes A22 -= L21 L21^T (a rank-CHOL_NB
+// update on MFMA).
+#include "chol.hpp"
+#include "gemm.hpp"
+
+namespace gmc {
+
+namespace {
+
+constexpr int LDSW = CHOL_NB + 1;   // padded column of the LDS blocks
+constexpr size_t DIAG_LDS = 2 * (size_t)CHOL_NB * LDSW * sizeof(double);
+
+// Factor of the kb x kb diagonal block at (k, k) and its inverse.  a[c][r]
+// holds block element (r, c), li[c][r] the inverse's.  A problem with
+// info != 0 is skipped.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k, int kb, double* Linv, int* info)
+{
+    extern __shared__ double lds[];
+    double (*a)[LDSW] = reinterpret_cast<double (*)[LDSW]>(lds);
+    double (*li)[LDSW] = reinterpret_cast<double (*)[LDSW]>(lds + CHOL_NB * LDSW);
+    __shared__ int bad;
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (info[b] != 0) return;
+    double* Ab = A + (size_t)b * n * n;
+    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+        const int r = idx % kb, c = idx / kb;
+        a[c][r] = r >= c ? Ab[(size_t)(k + c) * n + k + r] : 0.0;
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < kb; j++) {
+        const double p = a[j][j];   // the same value in every thread: the branch is uniform
+        if (!(p > 0.0 && p < INFINITY)) {
+            if (t == 0) bad = k + j + 1;
+            break;
+        }
+        const double l = sqrt(p);
+        __syncthreads();
+        for (int r = j + t; r < kb; r += blockDim.x) a[j][r] = r == j ? l : a[j][r] / l;
+        __syncthreads();
+        const int m = kb - 1 - j;
+        for (int idx = t; idx < m * m; idx += blockDim.x) {
+            const int r = j + 1 + idx % m, c = j + 1 + idx / m;
+            if (r >= c) a[c][r] -= a[j][r] * a[j][c];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (bad) {
+        if (t == 0) info[b] = bad;
+        return;
+    }
+    // inverse of the lower-triangular factor, column c by thread c
+    if (t < kb) {
+        const int c = t;
+        for (int r = 0; r < c; r++) li[c][r] = 0.0;
+        li[c][c] = 1.0 / a[c][c];
+        for (int r = c + 1; r < kb; r++) {
+            double s = 0.0;
+            for (int j = c; j < r; j++) s += a[j][r] * li[c][j];
+            li[c][r] = -s / a[r][r];
+        }
+    }
+    __syncthreads();
+    double* Lb = Linv + (size_t)b * CHOL_NB * CHOL_NB;
+    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+        const int r = idx % kb, c = idx / kb;
+        Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
+        Lb[(size_t)c * CHOL_NB + r] = li[c][r];
+    }
+}
+
+// P = A21: rows k + kb .. n-1, columns k .. k + kb - 1 (P: ld n, stride n CHOL_NB).
+__global__ void potrf_copy_panel_kernel(const double* A, int n, int k, int kb, double* P)
+{
+    const int b = blockIdx.y, m = n - k - kb;
+    const double* Ab = A + (size_t)b * n * n;
+    double* Pb = P + (size_t)b * n * CHOL_NB;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < m * kb; idx += gridDim.x * blockDim.x) {
+        const int r = idx % m, c = idx / m;
+        Pb[(size_t)c * n + r] = Ab[(size_t)(k + c) * n + k + kb + r];
+    }
+}
+
+__global__ void zero_strict_upper_kernel(double* A, int n)
+{
+    const int b = blockIdx.y;
+    double* Ab = A + (size_t)b * n * n;
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % n), c = (int)(idx / n);
+        if (r < c) Ab[idx] = 0.0;
+    }
+}
+
+__global__ void zero_info_kernel(int* info, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) info[b] = 0;
+}
+
+}  // namespace
+
+size_t potrf_work_bytes(int n, int B)
+{
+    return ((size_t)B * CHOL_NB * CHOL_NB + (size_t)B * n * CHOL_NB) * sizeof(double);
+}
+
+int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info)
+{
+    GMC_CHECK(n >= 1 && B >= 1, "potrf: n=%d, batch=%d", n, B);
+    double* Linv = static_cast<double*>(work);
+    double* P = Linv + (size_t)B * CHOL_NB * CHOL_NB;
+    const int64_t nn = (int64_t)n * n;
+    static bool attr = false;
+    if (!attr) {
+        GMC_HIP(hipFuncSetAttribute((const void*)potrf_diag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)DIAG_LDS));
+        attr = true;
+    }
+    hipLaunchKernelGGL(zero_info_kernel, dim3((B + 255) / 256), dim3(256), 0, s, info, B);
+    for (int k = 0; k < n; k += CHOL_NB) {
+        const int kb = std::min(CHOL_NB, n - k), m = n - k - kb;
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(B), dim3(256), DIAG_LDS, s, A, n, k, kb, Linv, info);
+        GMC_HIP(hipGetLastError());
+        if (m == 0) break;
+        const unsigned gp = (unsigned)std::min<int64_t>(4096, ((int64_t)m * kb + 255) / 256);
+        hipLaunchKernelGGL(potrf_copy_panel_kernel, dim3(gp, B), dim3(256), 0, s, A, n, k, kb, P);
+        GMC_HIP(hipGetLastError());
+        double* A21 = A + (k + kb) + (size_t)k * n;
+        // L21 = A21 L11^-T
+        GemmShape gp21{m, kb, kb, false, true, n, CHOL_NB, n, (int64_t)n * CHOL_NB, (int64_t)CHOL_NB * CHOL_NB, nn, B,
+                       0};
+        GMC_TRY(gemm_big<double>(s, gp21, 1.0, P, Linv, 0.0, A21));
+        // A22 -= L21 L21^T (lower triangle)
+        GemmShape g22{m, m, kb, false, true, n, n, n, nn, nn, nn, B, 1};
+        GMC_TRY(gemm_big<double>(s, g22, -1.0, A21, A21, 1.0, A21 + (size_t)kb * n));
+    }
+    const unsigned gz = (unsigned)std::min<int64_t>(65536, (nn + 255) / 256);
+    hipLaunchKernelGGL(zero_strict_upper_kernel, dim3(gz, B), dim3(256), 0, s, A, n);
+    GMC_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace gmc

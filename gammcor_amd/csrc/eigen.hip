@@ -1361,6 +1361,7 @@ int eig_bind(EigPlan<T>* p, void* ws)
 {
     GMC_TRY(eig_layout(p, static_cast<char*>(ws)));
     p->bound = ws != nullptr;
+    p->clear_err = true;
     return 0;
 }
 
@@ -1370,18 +1371,46 @@ static inline unsigned grid1(int64_t work, int per = 256, int64_t cap = 1 << 20)
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
-// The panel kernel needs its B*G workgroups (at most one per CU) resident
-// together.  A cooperative launch guarantees it; when the runtime refuses one
-// (its global wave-sync resource is held by another process on the device),
-// a plain launch of the same grid is used: with the grid no larger than the
-// CU count every workgroup is placed at once on an otherwise idle device, and
-// the kernel's bounded barrier spin turns a lost co-residency into an error
-// (reported by eig_check_error) instead of a hang.
+static bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
+// The group barriers need every workgroup of the grid resident at once; a
+// cooperative launch guarantees that, on a stream being captured into a graph
+// too (a kernel node with the cooperative attribute).  When the runtime
+// refuses the eager cooperative launch, a plain launch is taken only if the
+// occupancy calculator puts the whole grid on the device at once; otherwise
+// the call fails instead of waiting for a barrier that cannot complete.
 static int launch_persistent(const void* kfn, dim3 grid, dim3 block, void** args, size_t lds, hipStream_t s)
 {
-    hipError_t e = hipLaunchCooperativeKernel(kfn, grid, block, args, (unsigned)lds, s);
+    if (capturing(s)) {
+        hipLaunchAttribute attr{};
+        attr.id = hipLaunchAttributeCooperative;
+        attr.val.cooperative = 1;
+        hipLaunchConfig_t cfg{};
+        cfg.gridDim = grid;
+        cfg.blockDim = block;
+        cfg.dynamicSmemBytes = lds;
+        cfg.stream = s;
+        cfg.attrs = &attr;
+        cfg.numAttrs = 1;
+        const hipError_t e = hipLaunchKernelExC(&cfg, kfn, args);
+        GMC_CHECK(e == hipSuccess, "syevd: cooperative launch not captured into the graph (%s)", hipGetErrorString(e));
+        return 0;
+    }
+    const hipError_t e = hipLaunchCooperativeKernel(kfn, grid, block, args, (unsigned)lds, s);
     if (e == hipSuccess) return 0;
     (void)hipGetLastError();
+    int dev = 0, ncu = 0, per_cu = 0;
+    GMC_HIP(hipGetDevice(&dev));
+    GMC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    GMC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, (int)(block.x * block.y * block.z), lds));
+    const long resident = (long)per_cu * ncu, need = (long)grid.x * grid.y * grid.z;
+    GMC_CHECK(need <= resident,
+              "syevd: cooperative launch refused (%s) and %ld workgroups do not fit the %ld resident slots",
+              hipGetErrorString(e), need, resident);
     GMC_HIP(hipLaunchKernel(kfn, grid, block, args, lds, s));
     return 0;
 }
@@ -1394,7 +1423,10 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     const size_t nn = (size_t)n * n;
     hipLaunchKernelGGL(zero_u32, dim3(((B * G + 1) * BAR_STRIDE + 255) / 256), dim3(256), 0, s, p.bar,
                        (B * G + 1) * BAR_STRIDE);
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, (unsigned*)p.err, 16);
+    if (p.clear_err) {
+        hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, (unsigned*)p.err, 16);
+        p.clear_err = false;
+    }
     GMC_HIP(hipGetLastError());
 
     // ---------------- tridiagonalisation
@@ -1421,7 +1453,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
                                        : (const void*)sytrd_panel_kernel<T, true, false, KV, UV>)
                                : (plds ? (const void*)sytrd_panel_kernel<T, false, true, KV, UV>
                                        : (const void*)sytrd_panel_kernel<T, false, false, KV, UV>);
-        GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        // (set by an eager call before a capture; not repeated inside one)
+        if (!capturing(s)) GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned epoch = 0;
         for (int k = 0; k < n - 1; k += nb) {
             const int kb = std::min(nb, n - 1 - k);
@@ -1483,8 +1516,9 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     T* Tm = p.Tf + (size_t)B * nbb * nbb;
     if (n > 1) {
         const size_t lds_t = ((size_t)nbb * (nbb + 1) + nbb) * sizeof(T);
-        GMC_HIP(hipFuncSetAttribute((const void*)larft_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_t));
+        if (!capturing(s))
+            GMC_HIP(hipFuncSetAttribute((const void*)larft_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds_t));
         const int last = ((n - 2) / nbb) * nbb;
         for (int k = last; k >= 0; k -= nbb) {
             const int kb = std::min(nbb, n - 1 - k);
@@ -1521,6 +1555,7 @@ int eig_check_error(hipStream_t s, EigPlan<T>& p)
     int h = 0;
     GMC_HIP(hipMemcpyAsync(&h, p.err, sizeof(int), hipMemcpyDeviceToHost, s));
     GMC_HIP(hipStreamSynchronize(s));
+    if (h != 0) p.clear_err = true;
     GMC_CHECK(h == 0, "syevd: a workgroup barrier timed out (co-residency lost)");
     return 0;
 }

@@ -39,6 +39,9 @@ struct EigPlan {
     int *rotp = nullptr, *rotj = nullptr, *kcnt = nullptr, *rcnt = nullptr;
     unsigned* bar = nullptr;
     int* err = nullptr;
+    // the barrier time-out flag holds until read: the first solve after a
+    // bind clears it, later ones add to it (eig_check_error reads and clears)
+    bool clear_err = true;
     bool bound = false;
     bool owns_tables = true;   // d_desc / d_split allocated by eig_plan (else placed by eig_plan_at)
     bool own_gemm = false;     // back-transformation products on the library's MFMA tile only
@@ -68,7 +71,8 @@ int eig_bind(EigPlan<T>* p, void* ws);
 template <class T>
 int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w);
 
-// Synchronises the stream and reports a barrier time-out of the panel kernel.
+// Synchronises the stream and reports a barrier time-out of the panel kernel
+// in any solve since the plan was bound or last checked (the flag is cleared).
 template <class T>
 int eig_check_error(hipStream_t s, EigPlan<T>& p);
 

@@ -26,6 +26,10 @@ int gemm(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T b
 template <class T>
 int gemm_big(hipStream_t s, const GemmShape& g, T alpha, const T* A, const T* B, T beta, T* C);
 void blas_release(void* handle);
+// While on (per calling thread), gemm_big runs the library's own MFMA kernel
+// instead of rocBLAS (for sequences captured into a HIP graph).  Returns the
+// previous setting.
+bool set_own_gemm(bool on);
 
 // C (FP32) = alpha * op(A) * op(B) + beta * C with A, B in IEEE half
 // precision and FP32 accumulation (rocBLAS gemm_ex, full products only).

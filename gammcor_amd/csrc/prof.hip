@@ -52,6 +52,8 @@ void prof_release()
 ProfScope::ProfScope(hipStream_t st, const char* name, double work) : s(st)
 {
     if (!g_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) return;
     std::lock_guard<std::mutex> lk(g_mu);
     Rec r{name, take_event(), take_event(), work};
     if (!r.a || !r.b) return;

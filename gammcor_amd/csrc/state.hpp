@@ -23,6 +23,7 @@ struct State {
     Workspace io;          // host <-> device staging of host-pointer calls
     Workspace fofo;        // the (FF|OO) and (FO|FO) integral classes of the FOFO variant
     Workspace resolve;     // FP64 eigensolver of the mixed path's re-solves, reserved on first use
+    Workspace batch;       // the batching form: every system's A(0), A(1), weights; tables and outputs
     const void* resident_key = nullptr;   // host TwoNO array mirrored in `resident`
     int64_t resident_n = 0;
     double resident_probe[64] = {0};
@@ -41,7 +42,7 @@ struct State {
         eig32.clear();
         blas_release(blas);
         blas = nullptr;
-        for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo, &resolve})
+        for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo, &resolve, &batch})
             if (w->ptr) (void)hipFree(w->ptr);
         scratch = Workspace{};
         resident = Workspace{};
@@ -51,6 +52,7 @@ struct State {
         io = Workspace{};
         fofo = Workspace{};
         resolve = Workspace{};
+        batch = Workspace{};
         resident_key = nullptr;
         resident_n = 0;
         fofo_key[0] = fofo_key[1] = nullptr;

@@ -7,7 +7,9 @@ tensors on ``cuda:0``; torch only provides device memory and streams.
 """
 from __future__ import annotations
 
+import json
 import math
+import os
 
 import numpy as np
 
@@ -307,6 +309,64 @@ def ACECORR_host(twono, ure, occ, xone, indn, nbasis, ninact, nact, igem, rdm2ac
               ig.ctypes.data, xo.ctypes.data, _ptr(u), rd.ctypes.data, len(indn), indn.ctypes.data, na,
               al.ctypes.data, wt.ctypes.data, ec.ctypes.data, wa.ctypes.data, es.ctypes.data, nneg.ctypes.data)
     return dict(ecorr=float(ec[0]), walpha=wa, ecasscf=float(es[0]), nneg=nneg)
+
+
+# ---------------------------------------------------------------- batching form
+TUNING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "batch_gfx950.json")
+
+
+def batch_cost():
+    """The batch cost model {c0, c1, c2, c3} fitted by tune_batch.py (None
+    when no tuning file is present: the library's built-in fit)."""
+    if not os.path.exists(TUNING):
+        return None
+    with open(TUNING) as f:
+        return _cost4(json.load(f)["cost"])
+
+
+def _cost4(c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    if c.shape != (4,):
+        raise ValueError(f"batch cost model: 4 coefficients (c0 + c1 npad + c2 B npad^2 + c3 B npad^3), got {c.shape}")
+    return c
+
+
+def batch_plan(ndimx, nalpha, cost=None):
+    """(group per system, padded order per system) of the batching form."""
+    nd = np.ascontiguousarray(ndimx, dtype=np.int32)
+    grp = np.zeros(len(nd), dtype=np.int32)
+    npad = np.zeros(len(nd), dtype=np.int32)
+    c = batch_cost() if cost is None else _cost4(cost)
+    ng = _lib.lib().gmc_batch_plan(len(nd), nd.ctypes.data, nalpha, None if c is None else c.ctypes.data,
+                                   grp.ctypes.data, npad.ctypes.data)
+    _lib.check(0 if ng > 0 else ng)
+    return grp, npad
+
+
+def ACECORR_BATCH(systems, alphas, weights, prec=GMC_FP64, graph=False, cost=None, stream=None):
+    """The batching form: ACECORR for many systems (DeviceSystem list) with
+    all their alpha points in shared, padded batches.  Returns dict with
+    ecorr and ecasscf arrays (one per system)."""
+    import ctypes
+    n = len(systems)
+    P = ctypes.c_void_p * n
+    ints = lambda f: np.ascontiguousarray([f(d) for d in systems], dtype=np.int32)
+    nb, ni, na, nd = ints(lambda d: d.nbasis), ints(lambda d: d.ninact), ints(lambda d: d.nact), ints(lambda d: d.ndimx)
+    tw = P(*[d.twono.data_ptr() for d in systems])
+    oc = P(*[d.occ.data_ptr() for d in systems])
+    ig = P(*[d.igem.data_ptr() for d in systems])
+    xo = P(*[d.xone.data_ptr() for d in systems])
+    rd = P(*[d.rdm2act.data_ptr() for d in systems])
+    ix = P(*[d.indn.data_ptr() for d in systems])
+    al = np.ascontiguousarray(alphas, dtype=np.float64)
+    wt = np.ascontiguousarray(weights, dtype=np.float64)
+    c = batch_cost() if cost is None else _cost4(cost)
+    ec, es = np.zeros(n), np.zeros(n)
+    _lib.call("gmc_acecorr_batch_dev", prec, n, nb.ctypes.data, ni.ctypes.data, na.ctypes.data, nd.ctypes.data,
+              ctypes.addressof(tw), ctypes.addressof(oc), ctypes.addressof(ig), ctypes.addressof(xo),
+              ctypes.addressof(rd), ctypes.addressof(ix), len(al), al.ctypes.data, wt.ctypes.data,
+              None if c is None else c.ctypes.data, 1 if graph else 0, ec.ctypes.data, es.ctypes.data, stream)
+    return dict(ecorr=ec, ecasscf=es)
 
 
 # ---------------------------------------------------------------- SAPT

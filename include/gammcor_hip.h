@@ -208,6 +208,32 @@ int gmc_acecorr_fofo_dev(int prec, int alpha_batch, int nbasis, int ninact, int 
  * Replaces the DSYEVD behind Diag8 (misc.f:541-616) without its transpose. */
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
 
+/* ---- the batching form: many systems of uneven size ---------------------- */
+
+/* ACECORR (acfd.f:161-201) for nsys independent systems, each with all
+ * nalpha alpha points (the reference runs systems and alpha points one after
+ * another, accas.f:119, 192).  Per system i: nbasis[i], ninact[i], nact[i],
+ * ndimx[i] and device arrays twono[i], occ[i], igem[i], xone[i] (NO basis),
+ * rdm2act[i], indn[i] as gmc_acecorr_dev takes them (host arrays of device
+ * pointers).  The systems are assembled once, sorted by order and cut into
+ * groups whose systems x alpha points run as one batch padded with zeros to
+ * the group's largest order (the padded modes are decoupled and carry no
+ * weight, so every system's W(alpha) is that of its own solve).  The groups minimise
+ * cost = {c0, c1, c2, c3}: c0 + c1 npad + c2 B npad^2 + c3 B npad^3 ms per batch of B problems
+ * (NULL: the built-in fit of this GPU; gammcor_amd/tune_batch.py measures
+ * it).  prec GMC_FP64 or GMC_MIXED; graph != 0 (GMC_FP64) captures the group
+ * sequence into one HIP graph after an eager pass and replays it.  Out (host):
+ * ecorr[i], ecasscf[i] (may be NULL).  No counterpart in the reference. */
+int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* ninact, const int* nact,
+                          const int* ndimx, const double* const* twono, const double* const* occ,
+                          const int* const* igem, const double* const* xone, const double* const* rdm2act,
+                          const int* const* indn, int nalpha, const double* alphas, const double* weights,
+                          const double* cost, int graph, double* ecorr, double* ecasscf, void* stream);
+
+/* The grouping gmc_acecorr_batch_dev uses: group_of[i] and npad_of[i] per
+ * system (host arrays); returns the number of groups (negative on error). */
+int gmc_batch_plan(int nsys, const int* ndimx, int nalpha, const double* cost, int* group_of, int* npad_of);
+
 /* ---- SAPT: second-order dispersion from the same A+-B and ERPA ----------- */
 
 /* A monomer's ERPA response as SAPT computes it (sapt_main.f90:2593-2714):

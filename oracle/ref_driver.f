@@ -47,6 +47,7 @@ C     IFlCore=1, ThrSelAct=1e-8, ThrQVirt=1e-7 (types.f90 defaults).
      $ EGOne(:),UNOAO(:,:)
       Integer, Allocatable :: IndN(:,:),IndX(:),IndAux(:)
       Integer Hdr(5)
+      Integer(8) IC0,IC1,ICR
       Integer(8) NAddr3
       External NAddr3
       type(SystemBlock) :: System
@@ -123,7 +124,10 @@ C
       EndDo
       EndIf
 C
+      Call System_Clock(IC0,ICR)
       Call TwoNO1(TwoNO,UAOMO,NBasis,NInte2)
+      Call System_Clock(IC1)
+      Write(6,'(A,ES15.6)') 'TIME_TWONO1 ',Dble(IC1-IC0)/Dble(ICR)
       S1=0.D0
       S2=0.D0
       Do I=1,NInte2
@@ -248,6 +252,7 @@ C
 C
       ECorr=0.D0
       Do IA=1,NAl
+      Call System_Clock(IC0,ICR)
       If(Mode(1:4).Eq.'fofo') Then
       Call ACEInteg_FOFO(ECorrA,URe,Occ,XOne,UNOAO,
      $ ABPLUS,ABMIN,EigVecR,Eig,EGOne,NGOcc,CICoef,
@@ -261,6 +266,9 @@ C
      $ Title,NBasis,NInte1,NInte2,NDimX,NGem,IndAux,AG(IA),
      $ IndN,IndX,NDimX)
       EndIf
+      Call System_Clock(IC1)
+      Write(6,'(A,I4,ES15.6)') 'TIME_ACEINTEG ',IA,
+     $ Dble(IC1-IC0)/Dble(ICR)
       ECorr=ECorr+WG(IA)*ECorrA
       ES=0.D0
       NPos=0

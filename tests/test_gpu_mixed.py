@@ -29,9 +29,12 @@ def test_mixed_vs_fp64(cuda, n, ni, na, ne, rank, seed, nal):
 
 def test_unconverged_problems_are_solved_again_in_fp64(cuda, monkeypatch, capfd):
     """With the convergence threshold at zero no refinement converges: after
-    six steps every problem of both eigen solves is solved again in FP64, and
-    the mixed result is the FP64 path's.  At the default threshold the same
-    system converges in three steps and nothing is solved again."""
+    six steps every problem is solved again in FP64, and the mixed result is
+    the FP64 path's.  The default (Cholesky) route refines only A+'s
+    eigenpairs below zero, so only M's problems are solved again; the eigen
+    route (GMC_MIXED_ROUTE=eigen) refines A+ in full and solves both again.
+    At the default threshold the same system converges in three steps and
+    nothing is solved again."""
     from gammcor_amd import host, synth
     import accas_oracle as O
     s = synth.make_system(48, 6, 4, 4, 144, 0.1, 0)
@@ -49,11 +52,60 @@ def test_unconverged_problems_are_solved_again_in_fp64(cuda, monkeypatch, capfd)
     f = host.ACECORR(d, a, w, prec=host.GMC_MIXED, return_eig=True)
     err = capfd.readouterr().err
     print(err, flush=True)
+    assert "OA A+" not in err and "Cholesky route: restricted step" in err
+    assert "[gmc] OA M: 8 of 8 problems solved again in FP64" in err
+    assert err.count("[gmc] OA M step") == 6
+    assert abs(f["ecorr"] - ref["ecorr"]) < 1e-10
+    assert np.max(np.abs(f["eig"].cpu().numpy() - ref["eig"].cpu().numpy())) < 1e-10
+    monkeypatch.setenv("GMC_MIXED_ROUTE", "eigen")
+    f = host.ACECORR(d, a, w, prec=host.GMC_MIXED, return_eig=True)
+    err = capfd.readouterr().err
+    print(err, flush=True)
     assert "[gmc] OA A+: 8 of 8 problems solved again in FP64" in err
     assert "[gmc] OA M: 8 of 8 problems solved again in FP64" in err
     assert err.count("[gmc] OA M step") == 6
     assert abs(f["ecorr"] - ref["ecorr"]) < 1e-10
     assert np.max(np.abs(f["eig"].cpu().numpy() - ref["eig"].cpu().numpy())) < 1e-10
+
+
+@pytest.mark.parametrize("system", ["n48", "n160"])
+def test_cholesky_route_matches_eigen_route(cuda, monkeypatch, system):
+    """The Cholesky route (the default: |A+| = L L^T from the eigenpairs of
+    A+ below zero) and the reference's eigen route (S = |A+|^(1/2),
+    GMC_MIXED_ROUTE=eigen) give the same W(alpha), within 1 uEh of the FP64
+    path, and the same count of negative eigenvalues of A+, which is not
+    zero on these systems."""
+    from gammcor_amd import host, _lib
+    if system == "n48":
+        d, a, w = _n48_system()
+    else:
+        from gammcor_amd import synth
+        s = synth.config_system("n160_cas88", with_integrals=False)
+        import torch
+        ao = synth.ao_integrals_device(s.chol, cuda)
+        ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(cuda)
+        _lib.call("gmc_twono1_dev", ao.data_ptr(), ure.data_ptr(), s.nbasis, ao.numel(), None)
+        d = host.DeviceSystem(ao, s.occ, s.xone, s.gamma_act, s.nbasis, s.ninact, s.nact, s.nele, device=cuda)
+        a, w = host.GauLeg(0.0, 1.0, 8)
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64)
+    _lib.call("gmc_prof_reset")
+    _lib.call("gmc_prof_enable", 1)
+    try:
+        c = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+        routes_c = (_lib.prof_get("mixed_chol_route")[1], _lib.prof_get("mixed_eigen_route")[1])
+        monkeypatch.setenv("GMC_MIXED_ROUTE", "eigen")
+        _lib.call("gmc_prof_reset")
+        e = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+        routes_e = (_lib.prof_get("mixed_chol_route")[1], _lib.prof_get("mixed_eigen_route")[1])
+    finally:
+        _lib.call("gmc_prof_enable", 0)
+    print(f"{system}: |dE| Cholesky route {abs(c['ecorr'] - ref['ecorr']):.2e}, eigen route "
+          f"{abs(e['ecorr'] - ref['ecorr']):.2e}; nneg {list(c['nneg'])} / {list(ref['nneg'])}", flush=True)
+    assert routes_c == (1, 0) and routes_e == (0, 1)
+    assert np.all(np.asarray(ref["nneg"]) > 0)
+    assert np.array_equal(c["nneg"], ref["nneg"]) and np.array_equal(e["nneg"], ref["nneg"])
+    assert abs(c["ecorr"] - ref["ecorr"]) < 1e-6 and abs(e["ecorr"] - ref["ecorr"]) < 1e-6
+    assert np.max(np.abs(np.asarray(c["walpha"]) - np.asarray(ref["walpha"]))) < 1e-6
 
 
 def _n48_system():
