@@ -32,7 +32,7 @@ SIGNATURES = {
     "gmc_syevd_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
     "gmc_syevd_noalloc_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp],
     "gmc_acecorr_batch_dev": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp,
-                              _c_int, _vp, _vp, _vp],
+                              _c_int, _vp, _vp, _vp, _vp],
     "gmc_batch_plan": [_c_int, _vp, _c_int, _vp, _vp, _vp],
     "gmc_sapt_response_dev": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp,
                               _vp],

@@ -865,35 +865,65 @@ constexpr int OA_MAXPAIRS = 4096;
 constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
 constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
 
-__device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                          unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j);
+__device__ float2 oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+                           unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j, int wmode);
 
 __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
                                                    const double* delta, double* E, int n, unsigned* npair, int4* pairs,
-                                                   double2* rot, unsigned cap, unsigned* emax)
+                                                   double2* rot, unsigned cap, unsigned* emax, unsigned* pmax,
+                                                   int wmode)
 {
     const int b = blockIdx.z;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    // largest off-diagonal correction of the block, one atomic per block
-    // (oa_e_one returns +inf for a NaN element, so fmaxf keeps it)
-    __shared__ float wmax[4];
-    float my = 0.f;
-    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, cap, b, i, j);
-    for (int o = 32; o > 0; o >>= 1) my = fmaxf(my, __shfl_xor(my, o, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = my;
+    // largest off-diagonal correction and prediction of the block, one
+    // atomic each per block (oa_e_one returns +inf for a NaN element, so
+    // fmaxf keeps it)
+    __shared__ float wmax[4], wpred[4];
+    float2 my = make_float2(0.f, 0.f);
+    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode);
+    for (int o = 32; o > 0; o >>= 1) {
+        my.x = fmaxf(my.x, __shfl_xor(my.x, o, 64));
+        my.y = fmaxf(my.y, __shfl_xor(my.y, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = my.x;
+        wpred[threadIdx.x >> 6] = my.y;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        atomicMax(&emax[(size_t)b * OA_EMAX_SLOTS + (blockIdx.y % OA_EMAX_SLOTS)], __float_as_uint(m));
+        const size_t slot = (size_t)b * OA_EMAX_SLOTS + (blockIdx.y % OA_EMAX_SLOTS);
+        atomicMax(&emax[slot], __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+        atomicMax(&pmax[slot], __float_as_uint(fmaxf(fmaxf(wpred[0], wpred[1]), fmaxf(wpred[2], wpred[3]))));
     }
 }
 
-// One element of E; returns |E_ij| for i != j (0 on the diagonal), +inf
-// when it is not finite.  The separated-or-close test is symmetric in (i, j)
-// (the larger of |s_ij + l_j r_ij| and |s_ij + l_i r_ij|), so E_ij and E_ji
-// always take the same branch.
-__device__ float oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                          unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j)
+// The weight of eigenvalue l in what the solve makes of the eigenvectors:
+// wmode 1, sqrt|l| (S = V |L|^(1/2) V^T of A+); wmode 2, the ERPA integrand's
+// 1 / (2 omega) for a mode the filter keeps (omega^2 > 1e-6 and 1e-3 < omega
+// < 1e8, interpa.f:1748-1776, acfd.f:4976, 5013) and 0 for the others.
+__device__ inline double oa_weight(double l, int wmode)
+{
+    if (wmode == 1) return sqrt(fabs(l));
+    if (l <= 1e-6) return 0.0;
+    const double om = sqrt(l);
+    return om > 1e-3 && om < 1e8 ? 0.5 / om : 0.0;
+}
+
+// One element of E; returns (|E_ij|, the predicted next correction of the
+// pair) for i != j ((0, 0) on the diagonal), +inf when not finite.  The
+// separated-or-close test is symmetric in (i, j) (the larger of
+// |s_ij + l_j r_ij| and |s_ij + l_i r_ij|), so E_ij and E_ji always take the
+// same branch.  The step converges quadratically with a constant of about
+// max|l| / gap, so a pair with a gap small against the spectrum keeps an
+// error far above E_ij^2 after the step; a rotation within the pair moves
+// the result in proportion to how differently the solve weighs the two
+// eigenvalues (oa_weight).  The prediction of a separated pair is therefore
+// E_ij^2 (max|l| / |l_j - l_i|) |w_i - w_j| / max(w_i, w_j) (0 for wmode 0
+// or two weights of 0): large for an ERPA mode the filter keeps next to one
+// it drops (a gap of 1e-5 of the spectrum in one system of the 64-system
+// workload), small for two kept modes of nearly equal omega.
+__device__ float2 oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+                           unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j, int wmode)
 {
     const size_t base = (size_t)b * n * n;
     const int hi = i > j ? i : j, lo = i > j ? j : i;
@@ -903,9 +933,14 @@ __device__ float oa_e_one(const double* S, const double* G, const double* lam, c
     const double li = lam[(size_t)b * n + i], lj = lam[(size_t)b * n + j];
     const double num = sij + lj * rij, den = lj - li;
     const double nsym = fmax(fabs(num), fabs(sij + li * rij));
-    double e;
+    double e, pred = 0.0;
     if (i != j && fabs(den) > fmax(8.0 * nsym, delta[b])) {
         e = num / den;
+        if (wmode) {
+            const double wi = oa_weight(li, wmode), wj = oa_weight(lj, wmode), wm = fmax(wi, wj);
+            if (wm > 0.0)
+                pred = e * e * (delta[b] * (1.0 / (64.0 * 2.220446049250313e-16))) / fabs(den) * (fabs(wi - wj) / wm);
+        }
     } else {
         e = 0.5 * rij;
         if (i > j) {
@@ -925,9 +960,9 @@ __device__ float oa_e_one(const double* S, const double* G, const double* lam, c
         }
     }
     E[base + i + (size_t)j * n] = e;
-    if (i == j) return 0.f;
-    const float r = (float)fabs(e);
-    return r <= 3.4028235e38f ? r : INFINITY;
+    if (i == j) return make_float2(0.f, 0.f);
+    const float r = (float)fabs(e), q = (float)pred;
+    return make_float2(r <= 3.4028235e38f ? r : INFINITY, q <= 3.4028235e38f ? q : INFINITY);
 }
 
 // x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
@@ -976,31 +1011,35 @@ struct OaPairs {
     int4* pairs;        // B x OA_MAXPAIRS
     double2* rot;       // B x OA_MAXPAIRS
     unsigned* emax;     // B x OA_EMAX_SLOTS: float bits of max |E_ij| (i != j) of the last step
+    unsigned* pmax;     // B x OA_EMAX_SLOTS: float bits of the largest predicted next correction
     unsigned cap = OA_MAXPAIRS;   // pairs recorded per problem (<= OA_MAXPAIRS)
 };
 
-// Xn = X + P, P the FP32 product X E of the update.
-__global__ void oa_add_kernel(const double* X, const float* P, double* Xn, int64_t count)
+// Xn = X + s_b P, s_b the scale of problem b = i / nn (1 without scales).
+__global__ void oa_add_kernel(const double* X, const float* P, double* Xn, int64_t count, int64_t nn,
+                              const double* s)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        Xn[i] = X[i] + (double)P[i];
+        Xn[i] = X[i] + (s ? s[i / nn] : 1.0) * (double)P[i];
 }
 
-// X and E / scale in IEEE half precision (|X_ij| <= 1; E scaled to |E| <= ~1).
-__global__ void f64_to_f16_kernel(const double* a, __half* b, int64_t count, double scale)
+// X, and E / s_b per problem, in IEEE half precision (|X_ij| <= 1; E scaled
+// to |E| <= ~1).  inv: 1 / s_b per problem (NULL: no scaling).
+__global__ void f64_to_f16_kernel(const double* a, __half* b, int64_t count, int64_t nn, const double* inv)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = __float2half((float)(a[i] * scale));   // via float: 2^-24 against half's 2^-11
+        b[i] = __float2half((float)(a[i] * (inv ? inv[i / nn] : 1.0)));   // via float: 2^-24 against half's 2^-11
 }
 
-// hi + 2^-11 lo split of a * scale in IEEE half precision: the lo part carries
+// hi + 2^-11 lo split of a / s_b in IEEE half precision: the lo part carries
 // the next 11 bits, scaled by 2^11 so it stays in half's normal range; the
 // pair represents the value to 2^-22 relative (1.5e-11 absolute below half's
 // normal range).
-__global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int64_t count, double scale)
+__global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int64_t count, int64_t nn,
+                                    const double* inv)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
-        const float v = (float)(a[i] * scale);
+        const float v = (float)(a[i] * (inv ? inv[i / nn] : 1.0));
         const __half h = __float2half(v);
         hi[i] = h;
         lo[i] = __float2half((v - __half2float(h)) * 2048.0f);
@@ -1017,9 +1056,10 @@ __global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int
 // 2^-24) on the half MFMA rate.  The residual terms (A X, X^T A X, X^T X) are FP64.
 // pe receives each problem's correction of the step (oa_effective: +inf for
 // a non-finite step or more close pairs than recorded).
-static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, double*& Xn, double* lam, double* AX,
+static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, double*& X, double*& Xn, double* lam,
+                     double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
-                     float* E32, float* P32, float half_from, std::vector<float>& pe)
+                     float* E32, float* P32, float half_from, double* dscale, std::vector<float>& pe)
 {
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
@@ -1031,57 +1071,80 @@ static int oa_refine(hipStream_t st, int n, int B, const double* A, double*& X, 
     hipLaunchKernelGGL(oa_delta_kernel, dim3(1), dim3(64), 0, st, partial, npart, delta, B);
     GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
     GMC_HIP(hipMemsetAsync(op.emax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
+    GMC_HIP(hipMemsetAsync(op.pmax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
-                       op.pairs, op.rot, op.cap, op.emax);
+                       op.pairs, op.rot, op.cap, op.emax, op.pmax, wmode);
     GMC_HIP(hipGetLastError());
-    // each problem's largest off-diagonal correction (non-negative floats
-    // order as their bits; +inf marks a non-finite element) and pair count
-    std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS), np(B);
+    // each problem's largest off-diagonal correction and prediction
+    // (non-negative floats order as their bits; +inf marks a non-finite
+    // element) and pair count
+    std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS), pm((size_t)B * OA_EMAX_SLOTS), np(B);
     GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipMemcpyAsync(pm.data(), op.pmax, pm.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipMemcpyAsync(np.data(), op.npair, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipStreamSynchronize(st));
     pe.assign(B, 0.f);
-    // the scale of the update: the largest finite correction of the batch (a
-    // problem at +inf is solved again or fails the call, its update is moot)
-    float emax = 0.f;
+    // Each problem's update is scaled by its own largest correction: one
+    // batch holds problems at every stage of convergence (the batching form
+    // mixes systems), and a batch-wide scale would push a converging
+    // problem's small corrections below half's normal range.  A problem at
+    // +inf is solved again or fails the call; its update is moot.
+    std::vector<float> eb(B);
+    float emin = INFINITY;
     for (int b = 0; b < B; b++) {
-        unsigned mxb = 0;
-        for (int k = 0; k < OA_EMAX_SLOTS; k++) mxb = std::max(mxb, em[(size_t)b * OA_EMAX_SLOTS + k]);
-        float e;
+        unsigned mxb = 0, pxb = 0;
+        for (int k = 0; k < OA_EMAX_SLOTS; k++) {
+            mxb = std::max(mxb, em[(size_t)b * OA_EMAX_SLOTS + k]);
+            pxb = std::max(pxb, pm[(size_t)b * OA_EMAX_SLOTS + k]);
+        }
+        float e, q;
         std::memcpy(&e, &mxb, 4);
-        if (e <= 3.4028235e38f) emax = std::max(emax, e);
-        pe[b] = oa_effective(e, np[b], op.cap);
+        std::memcpy(&q, &pxb, 4);
+        pe[b] = oa_effective(oa_measure(e, q), np[b], op.cap);
+        eb[b] = e <= 3.4028235e38f ? e : 1.0f;
+        if (e <= 3.4028235e38f) emin = std::min(emin, e);
     }
+    // plain half inputs only when every problem's correction is large enough
+    // for their rounding (about 4.9e-4 max|E|) to stay under a tenth of the
+    // max|E|^2 the step leaves; else the split form for the whole batch
+    const bool plain = emin >= half_from;
+    std::vector<double> sc(2 * (size_t)B);
+    for (int b = 0; b < B; b++) {
+        // the split form's floor 1e-4: the diagonal E_ii = r_ii / 2 (not in
+        // max|E|) is a few 1e-6 at most from the FP32 start on (orthogonality
+        // ~3e-6), so E / s stays far inside half's range
+        const double sb = plain ? (eb[b] > 0.f ? eb[b] : 1.0) : std::max(eb[b], 1e-4f);
+        sc[b] = sb;
+        sc[B + b] = 1.0 / sb;
+    }
+    GMC_HIP(hipMemcpyAsync(dscale, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    const double* s_b = dscale;
+    const double* inv_b = dscale + B;
     const unsigned gcopy = (unsigned)std::min<int64_t>(65536, ((int64_t)B * nn + 255) / 256);
-    if (emax >= half_from) {
-        // half inputs in the FP32 buffers' space, E scaled by 1 / emax
+    const int64_t cnt = (int64_t)B * nn, snn = (int64_t)nn;
+    if (plain) {
+        // half inputs in the FP32 buffers' space, E scaled by 1 / s_b
         __half* X16 = reinterpret_cast<__half*>(X32);
         __half* E16 = reinterpret_cast<__half*>(E32);
-        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, X, X16, (int64_t)B * nn, 1.0);
-        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, AX, E16, (int64_t)B * nn,
-                           1.0 / (double)emax);
+        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, X, X16, cnt, snn, nullptr);
+        hipLaunchKernelGGL(f64_to_f16_kernel, dim3(gcopy), dim3(256), 0, st, AX, E16, cnt, snn, inv_b);
         GMC_HIP(hipGetLastError());
-        GMC_TRY(gemm_f16(st, gnn, emax, X16, E16, 0.0f, P32));
+        GMC_TRY(gemm_f16(st, gnn, 1.0f, X16, E16, 0.0f, P32));
     } else {
-        // hi and lo halves of X in X32's space, of E / den in E32's.  den is
-        // at least 1e-4: the diagonal E_ii = r_ii / 2 (not in emax) is a few
-        // 1e-6 at most from the FP32 start on (orthogonality ~3e-6), so E / den
-        // stays far inside half's range.
-        const float den = std::max(emax, 1e-4f);
+        // hi and lo halves of X in X32's space, of E / s_b in E32's
         __half* Xh = reinterpret_cast<__half*>(X32);
         __half* Xl = Xh + (size_t)B * nn;
         __half* Eh = reinterpret_cast<__half*>(E32);
         __half* El = Eh + (size_t)B * nn;
-        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, X, Xh, Xl, (int64_t)B * nn, 1.0);
-        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, AX, Eh, El, (int64_t)B * nn,
-                           1.0 / (double)den);
+        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, X, Xh, Xl, cnt, snn, nullptr);
+        hipLaunchKernelGGL(f64_to_f16x2_kernel, dim3(gcopy), dim3(256), 0, st, AX, Eh, El, cnt, snn, inv_b);
         GMC_HIP(hipGetLastError());
-        const float lo = den / 2048.0f;
-        GMC_TRY(gemm_f16(st, gnn, den, Xh, Eh, 0.0f, P32));
+        const float lo = 1.0f / 2048.0f;
+        GMC_TRY(gemm_f16(st, gnn, 1.0f, Xh, Eh, 0.0f, P32));
         GMC_TRY(gemm_f16(st, gnn, lo, Xh, El, 1.0f, P32));
         GMC_TRY(gemm_f16(st, gnn, lo, Xl, Eh, 1.0f, P32));
     }
-    hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, (int64_t)B * nn);
+    hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, cnt, snn, s_b);
     GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
     hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot, op.cap);
@@ -1105,7 +1168,8 @@ size_t erpa_mixed_bytes(int n, int batch)
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
                         B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4,
-                        B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8, potrf_work_bytes(n, batch), B * 4});
+                        B * (size_t)OA_EMAX_SLOTS * 4, B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8,
+                        potrf_work_bytes(n, batch), B * 4, 2 * B * 8});
 }
 
 // ------------------------------------------------------------ Cholesky route
@@ -1152,8 +1216,11 @@ __global__ void oac_lam_kernel(const double* S, const double* G, const float* w3
 // E_ij of the restricted step (i < n, j < K): (s_ij - l_j g_ij) / (l_j - l_i)
 // for separated pairs, -g_ij / 2 for close ones (the same rule as the full
 // step), (1 - g_jj) / 2 on the diagonal; l_i of an unrefined column is its
-// FP32 eigenvalue.  emax[b]: the largest |E_ij| off the diagonal, as float
-// bits (+inf for a non-finite element).
+// FP32 eigenvalue.  emax[b]: the largest |E_ij| off the diagonal over the
+// columns j with l_j < 0, the only ones |A+| takes (the others are refined
+// along but need not converge: a positive column next to the unrefined part
+// of the spectrum can converge slowly and does not matter), as float bits
+// (+inf for a non-finite element).
 __global__ __launch_bounds__(256) void oac_e_kernel(const double* S, const double* G, const double* lamJ,
                                                     const float* w32, const double* delta, int n, int K, double* E,
                                                     unsigned* emax)
@@ -1173,7 +1240,7 @@ __global__ __launch_bounds__(256) void oac_e_kernel(const double* S, const doubl
             const double gap = lj - li;
             e = (s - lj * g) / gap;
             if (!(fabs(gap) > delta[b]) || !(fabs(e) < 0.125)) e = -0.5 * g;
-            ea = isfinite(e) ? (float)fabs(e) : INFINITY;
+            if (!(lj >= 0.0)) ea = isfinite(e) ? (float)fabs(e) : INFINITY;
         }
         E[o] = e;
     }
@@ -1202,18 +1269,31 @@ __global__ void oac_reflect_kernel(const double* X, const double* lamJ, int n, i
     if (i == 0 && l < 0.0) atomicAdd(&nneg[b], 1);
 }
 
-// The batching form: a padded mode has a zero row and column in |A+|; a unit
-// diagonal there keeps L a factor of the system's own block (and the pad
-// block of M = L^T A- L zero).
-__global__ void pad_diag_one_kernel(const BatchSys* sys, int nalpha, double* A, int npad)
+// The batching form in the mixed path: the padded modes of A+ and of M get
+// diagonal values apart from the system's spectrum, so that the FP32
+// eigensolves, whose close-eigenvalue rotations act at FP32 rounding (about
+// 1e-7 of the norm), cannot mix them with the system's modes near zero.
+// A+'s pads take +max_i A+_ii and M's pads -max_i M_ii (i over the system's
+// own order): neither exceeds the largest eigenvalue, so the deflation
+// tolerances (8 eps max|d|) stay as they are.  With the pad value c in A+,
+// |A+| has c on its pad diagonal, its factor sqrt(c), and M's pad block is
+// zero before its pads are set.  The pad modes keep zero weight in the
+// integrand: only the system's block is read, and M's pads have omega^2 < 0,
+// which the filter drops.  One workgroup (256 threads) per problem.
+__global__ void pad_diag_set_kernel(const BatchSys* sys, int nalpha, double* A, int npad, double sign)
 {
-    const int k = blockIdx.y;
+    __shared__ double red[4];
+    const int b = blockIdx.x, k = b / nalpha;
     const int n = sys[k].n;
-    for (int a = 0; a < nalpha; a++) {
-        double* Ab = A + ((size_t)k * nalpha + a) * npad * npad;
-        for (int i = n + blockIdx.x * blockDim.x + threadIdx.x; i < npad; i += gridDim.x * blockDim.x)
-            Ab[(size_t)i * npad + i] = 1.0;
-    }
+    double* Ab = A + (size_t)b * npad * npad;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = fmax(mx, Ab[(size_t)i * npad + i]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (!(mx > 0.0)) mx = 1.0;
+    for (int i = n + threadIdx.x; i < npad; i += blockDim.x) Ab[(size_t)i * npad + i] = sign * mx;
 }
 
 // Mixed ERPA: eigen solves of A+ and of M in FP32.  A+ gives the factor F
@@ -1254,12 +1334,14 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     float* X32 = ar.take<float>(B * nn);
     float* P32 = ar.take<float>(B * nn);
     op.emax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
+    op.pmax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
     // GMC_OA_MAXPAIRS lowers the number of close pairs recorded per problem
     // (diagnostics: forces the over-the-cap path)
     if (const char* e = getenv("GMC_OA_MAXPAIRS")) op.cap = (unsigned)std::min(std::max(0, atoi(e)), OA_MAXPAIRS);
     double* wre = ar.take<double>(B * n);
     char* pwork = ar.take<char>(potrf_work_bytes(n, B));
     int* info = ar.take<int>(B);
+    double* dscale = ar.take<double>(2 * (size_t)B);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
     auto report = [&](const char* what, int it) -> int {
@@ -1288,12 +1370,19 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     // leaves unconverged go to `left`.  Without one, a problem whose
     // corrections are not finite or whose close pairs overflowed the record
     // fails the call.
-    auto refine = [&](const char* what, const double* A, double* lamv, double* axbuf, std::vector<int>& left) -> int {
+    auto refine = [&](const char* what, int wmode, const double* A, double* lamv, double* axbuf,
+                      std::vector<int>& left) -> int {
         left.clear();
         for (int it = 0; it < n_max; it++) {
-            GMC_TRY(oa_refine(st, n, B, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32, P32,
-                              half_from, pe));
+            GMC_TRY(oa_refine(st, n, B, wmode, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32,
+                              P32, half_from, dscale, pe));
             GMC_TRY(report(what, it));
+            if (stats && atoi(getenv("GMC_OA_STATS")) >= 2) {
+                // every problem's correction of the step
+                fprintf(stderr, "[gmc] OA %s step %d per problem:", what, it);
+                for (int b = 0; b < B; b++) fprintf(stderr, " %.1e", pe[b]);
+                fprintf(stderr, "\n");
+            }
             const OaDecision d = oa_decide(pe, it, n_min, n_max, OA_TOL, oa_extend, OA_GIVEUP, resolve != nullptr);
             if (d.next == OaNext::Continue) continue;
             if (d.next == OaNext::Resolve) left = d.left;
@@ -1334,6 +1423,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         // the batching form: alphas are already on the device
         hipLaunchKernelGGL(form_pad_kernel<double>, dim3(nt64f * (nt64f + 1) / 2, bt->nsys), blk, 0, st, bt->sys,
                            alphas, nalpha, Ap, Am, n);
+        hipLaunchKernelGGL(pad_diag_set_kernel, dim3(B), blk, 0, st, bt->sys, nalpha, Ap, n, 1.0);
     } else {
         GMC_HIP(hipMemcpyAsync(d_al, alphas, nalpha * sizeof(double), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(form_kernel<double>, dim3(nt64f * (nt64f + 1) / 2), blk, 0, st, ab.p0, ab.p1, ab.m0,
@@ -1393,9 +1483,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         GMC_HIP(hipMemcpyAsync(AX, Ap, (size_t)B * nn * sizeof(double), hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(oac_reflect_kernel, gk, blk, 0, st, X, lam, n, K, Sb, out.nneg);
         GemmShape gR{n, n, K, false, true, n, n, n, snn, snn, snn, B, 1};
-        GMC_TRY(gemm_big<double>(st, gR, 1.0, Sb, X, 1.0, AX));
-        if (bt) hipLaunchKernelGGL(pad_diag_one_kernel, dim3(std::max(1, (n + 255) / 256), bt->nsys), blk, 0, st,
-                                   bt->sys, nalpha, AX, n);
+        GMC_TRY(gemm<double>(st, gR, 1.0, Sb, X, 1.0, AX));   // rank K: one lower-triangle launch
         GMC_HIP(hipGetLastError());
         GMC_TRY(potrf_batched(st, n, B, AX, pwork, info));
         GMC_HIP(hipMemcpyAsync(hc.data(), info, B * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1416,7 +1504,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     if (!chol_ok) {
         // the eigen route of the reference: A+ refined in full, S = V|L|^(1/2)V^T (into AX)
         ProfScope ps(st, "mixed_eigen_route", 0.0);
-        GMC_TRY(refine("A+", Ap, lam, AX, left));
+        GMC_TRY(refine("A+", 1, Ap, lam, AX, left));
         GMC_TRY(resolve_left("A+", left, Ap, lam));
         hipLaunchKernelGGL(sqrt_scale64_kernel, gsq, blk, 0, st, X, lam, n, out.nneg);
         GMC_HIP(hipGetLastError());
@@ -1431,14 +1519,31 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     GemmShape gTl{n, n, n, true, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
     GMC_TRY(gemm_big<double>(st, gTl, 1.0, AX, Sb, 0.0, Ap));   // M is symmetric: its lower triangle, mirrored
     hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, Ap, n);
+    if (bt) hipLaunchKernelGGL(pad_diag_set_kernel, dim3(B), blk, 0, st, bt->sys, nalpha, Ap, n, -1.0);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(gcopy), blk, 0, st, Ap, E32, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     GMC_TRY(syevd_batched<float>(st, ep, E32, w32));
     hipLaunchKernelGGL(f32_to_f64_kernel, dim3(gcopy), blk, 0, st, E32, X, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
-    GMC_TRY(refine("M", Ap, om2, Am, left));
+    GMC_TRY(refine("M", 2, Ap, om2, Am, left));
     GMC_TRY(resolve_left("M", left, Ap, om2));
+    if (stats && atoi(getenv("GMC_OA_STATS")) >= 2) {
+        // per problem: modes the integrand keeps (omega^2 > 1e-6, 1e-3 < omega < 1e8)
+        std::vector<double> h((size_t)B * n);
+        GMC_HIP(hipMemcpyAsync(h.data(), om2, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        fprintf(stderr, "[gmc] M kept modes per problem:");
+        for (int b = 0; b < B; b++) {
+            int c = 0;
+            for (int i = 0; i < n; i++) {
+                const double o = h[(size_t)b * n + i];
+                c += o > 1e-6 && sqrt(o) > 1e-3 && sqrt(o) < 1e8;
+            }
+            fprintf(stderr, " %d", c);
+        }
+        fprintf(stderr, "\n");
+    }
     // Y = F Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
     GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
     hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);

@@ -102,7 +102,8 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
                           const int* ndimx, const double* const* twono, const double* const* occ,
                           const int* const* igem, const double* const* xone, const double* const* rdm2act,
                           const int* const* indn, int nalpha, const double* alphas, const double* weights,
-                          const double* cost, int graph, double* ecorr, double* ecasscf, void* stream)
+                          const double* cost, int graph, double* ecorr, double* ecasscf, double* walpha,
+                          void* stream)
 {
     GMC_TRY(ensure_init());
     GMC_CHECK(nsys >= 1 && nalpha >= 1, "ACECORR batch: bad sizes");
@@ -280,7 +281,10 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
         for (int k = 0; k < ng; k++) {
             const int sys = order[groups[g].begin + k];
             double e = 0.0;
-            for (int a = 0; a < nalpha; a++) e += weights[a] * W[(size_t)k * nalpha + a];
+            for (int a = 0; a < nalpha; a++) {
+                e += weights[a] * W[(size_t)k * nalpha + a];
+                if (walpha) walpha[(size_t)sys * nalpha + a] = W[(size_t)k * nalpha + a];
+            }
             ecorr[sys] = e;
             if (ecasscf) ecasscf[sys] = runs[sys].ab.ecasscf;
         }

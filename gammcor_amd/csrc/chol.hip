@@ -2,11 +2,12 @@
 // CHOL_NB columns).  The mixed ERPA path factors |A+| with it (the reference
 // takes |A+|^(1/2) from an eigendecomposition instead, interpa.f:1693-1716).
 //
-// Per panel k: one workgroup per problem factors the diagonal block in LDS
-// and inverts its factor (forward substitution, one column per thread); the
+// Per panel k: one workgroup per problem factors the diagonal block in LDS,
+// writes it back and inverts it in place (column by column from the right,
+// each column a triangular product with the part already inverted); the
 // panel below becomes L21 = A21 L11^-T (a product with the small inverse)
-// and the trailing lower triangle takes A22 -= L21 L21^T (a rank-CHOL_NB
-// update on MFMA).
+// and the trailing lower triangle takes A22 -= L21 L21^T, both on the
+// library's own MFMA tile (the trailing update as one lower-triangle launch).
 #include "chol.hpp"
 #include "gemm.hpp"
 
@@ -14,17 +15,17 @@ namespace gmc {
 
 namespace {
 
-constexpr int LDSW = CHOL_NB + 1;   // padded column of the LDS blocks
-constexpr size_t DIAG_LDS = 2 * (size_t)CHOL_NB * LDSW * sizeof(double);
+constexpr int LDSW = CHOL_NB + 1;   // padded column of the LDS block
+constexpr size_t DIAG_LDS = ((size_t)CHOL_NB * LDSW + CHOL_NB) * sizeof(double);
 
-// Factor of the kb x kb diagonal block at (k, k) and its inverse.  a[c][r]
-// holds block element (r, c), li[c][r] the inverse's.  A problem with
+// Factor of the kb x kb diagonal block at (k, k), written back, and its
+// inverse into Linv.  a[c][r] holds block element (r, c).  A problem with
 // info != 0 is skipped.
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k, int kb, double* Linv, int* info)
 {
     extern __shared__ double lds[];
     double (*a)[LDSW] = reinterpret_cast<double (*)[LDSW]>(lds);
-    double (*li)[LDSW] = reinterpret_cast<double (*)[LDSW]>(lds + CHOL_NB * LDSW);
+    double* tmp = lds + CHOL_NB * LDSW;
     __shared__ int bad;
     const int b = blockIdx.x, t = threadIdx.x;
     if (info[b] != 0) return;
@@ -57,23 +58,29 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
         if (t == 0) info[b] = bad;
         return;
     }
-    // inverse of the lower-triangular factor, column c by thread c
-    if (t < kb) {
-        const int c = t;
-        for (int r = 0; r < c; r++) li[c][r] = 0.0;
-        li[c][c] = 1.0 / a[c][c];
-        for (int r = c + 1; r < kb; r++) {
+    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+        const int r = idx % kb, c = idx / kb;
+        Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
+    }
+    // in-place inverse, columns from the right: column j of L^-1 below the
+    // diagonal is -(1/l_jj) Linv[j+1:, j+1:] l[j+1:, j]
+    for (int j = kb - 1; j >= 0; j--) {
+        __syncthreads();
+        const double djj = 1.0 / a[j][j];
+        for (int i = j + 1 + t; i < kb; i += blockDim.x) tmp[i] = a[j][i];
+        __syncthreads();
+        for (int i = j + 1 + t; i < kb; i += blockDim.x) {
             double s = 0.0;
-            for (int j = c; j < r; j++) s += a[j][r] * li[c][j];
-            li[c][r] = -s / a[r][r];
+            for (int q = j + 1; q <= i; q++) s += a[q][i] * tmp[q];
+            a[j][i] = -djj * s;
         }
+        if (t == 0) a[j][j] = djj;
     }
     __syncthreads();
     double* Lb = Linv + (size_t)b * CHOL_NB * CHOL_NB;
     for (int idx = t; idx < kb * kb; idx += blockDim.x) {
         const int r = idx % kb, c = idx / kb;
-        Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
-        Lb[(size_t)c * CHOL_NB + r] = li[c][r];
+        Lb[(size_t)c * CHOL_NB + r] = r >= c ? a[c][r] : 0.0;
     }
 }
 
@@ -138,10 +145,10 @@ int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info)
         // L21 = A21 L11^-T
         GemmShape gp21{m, kb, kb, false, true, n, CHOL_NB, n, (int64_t)n * CHOL_NB, (int64_t)CHOL_NB * CHOL_NB, nn, B,
                        0};
-        GMC_TRY(gemm_big<double>(s, gp21, 1.0, P, Linv, 0.0, A21));
-        // A22 -= L21 L21^T (lower triangle)
+        GMC_TRY(gemm<double>(s, gp21, 1.0, P, Linv, 0.0, A21));
+        // A22 -= L21 L21^T (lower triangle, one launch)
         GemmShape g22{m, m, kb, false, true, n, n, n, nn, nn, nn, B, 1};
-        GMC_TRY(gemm_big<double>(s, g22, -1.0, A21, A21, 1.0, A21 + (size_t)kb * n));
+        GMC_TRY(gemm<double>(s, g22, -1.0, A21, A21, 1.0, A21 + (size_t)kb * n));
     }
     const unsigned gz = (unsigned)std::min<int64_t>(65536, (nn + 255) / 256);
     hipLaunchKernelGGL(zero_strict_upper_kernel, dim3(gz, B), dim3(256), 0, s, A, n);

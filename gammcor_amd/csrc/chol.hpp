@@ -4,7 +4,7 @@
 
 namespace gmc {
 
-constexpr int CHOL_NB = 64;   // panel width
+constexpr int CHOL_NB = 128;   // panel width
 
 // Bytes of the work space potrf_batched needs for B problems of order n.
 size_t potrf_work_bytes(int n, int B);

@@ -346,7 +346,8 @@ def batch_plan(ndimx, nalpha, cost=None):
 def ACECORR_BATCH(systems, alphas, weights, prec=GMC_FP64, graph=False, cost=None, stream=None):
     """The batching form: ACECORR for many systems (DeviceSystem list) with
     all their alpha points in shared, padded batches.  Returns dict with
-    ecorr and ecasscf arrays (one per system)."""
+    ecorr and ecasscf arrays (one per system) and walpha (systems x alpha
+    points)."""
     import ctypes
     n = len(systems)
     P = ctypes.c_void_p * n
@@ -361,12 +362,13 @@ def ACECORR_BATCH(systems, alphas, weights, prec=GMC_FP64, graph=False, cost=Non
     al = np.ascontiguousarray(alphas, dtype=np.float64)
     wt = np.ascontiguousarray(weights, dtype=np.float64)
     c = batch_cost() if cost is None else _cost4(cost)
-    ec, es = np.zeros(n), np.zeros(n)
+    ec, es, wa = np.zeros(n), np.zeros(n), np.zeros((n, len(al)))
     _lib.call("gmc_acecorr_batch_dev", prec, n, nb.ctypes.data, ni.ctypes.data, na.ctypes.data, nd.ctypes.data,
               ctypes.addressof(tw), ctypes.addressof(oc), ctypes.addressof(ig), ctypes.addressof(xo),
               ctypes.addressof(rd), ctypes.addressof(ix), len(al), al.ctypes.data, wt.ctypes.data,
-              None if c is None else c.ctypes.data, 1 if graph else 0, ec.ctypes.data, es.ctypes.data, stream)
-    return dict(ecorr=ec, ecasscf=es)
+              None if c is None else c.ctypes.data, 1 if graph else 0, ec.ctypes.data, es.ctypes.data,
+              wa.ctypes.data, stream)
+    return dict(ecorr=ec, ecasscf=es, walpha=wa)
 
 
 # ---------------------------------------------------------------- SAPT

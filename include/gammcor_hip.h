@@ -223,12 +223,14 @@ int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream);
  * (NULL: the built-in fit of this GPU; gammcor_amd/tune_batch.py measures
  * it).  prec GMC_FP64 or GMC_MIXED; graph != 0 (GMC_FP64) captures the group
  * sequence into one HIP graph after an eager pass and replays it.  Out (host):
- * ecorr[i], ecasscf[i] (may be NULL).  No counterpart in the reference. */
+ * ecorr[i], ecasscf[i] (may be NULL), walpha[i * nalpha + a] = W(alpha_a) of
+ * system i (may be NULL).  No counterpart in the reference. */
 int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* ninact, const int* nact,
                           const int* ndimx, const double* const* twono, const double* const* occ,
                           const int* const* igem, const double* const* xone, const double* const* rdm2act,
                           const int* const* indn, int nalpha, const double* alphas, const double* weights,
-                          const double* cost, int graph, double* ecorr, double* ecasscf, void* stream);
+                          const double* cost, int graph, double* ecorr, double* ecasscf, double* walpha,
+                          void* stream);
 
 /* The grouping gmc_acecorr_batch_dev uses: group_of[i] and npad_of[i] per
  * system (host arrays); returns the number of groups (negative on error). */

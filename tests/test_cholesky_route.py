@@ -35,8 +35,9 @@ def gold(name):
 def partial_refine(a, x32, w32, k, steps=8, tol=1e-11):
     """Column-restricted Ogita-Aishima: the first k columns of the FP32
     eigenvector matrix x32 refined in FP64; the other columns and their FP32
-    eigenvalues serve as the fixed basis.  Stops at max |E| < tol (the
-    rounding floor of E is about 1e-13 here).  Returns (X_J, lambda_J, history of
+    eigenvalues serve as the fixed basis.  Stops when max |E| over the
+    columns with a negative eigenvalue is below tol (the rounding floor of E
+    is about 1e-13 here).  Returns (X_J, lambda_J, history of
     max |E| off the diagonal)."""
     n = a.shape[0]
     x = x32.astype(np.float64).copy()
@@ -57,7 +58,8 @@ def partial_refine(a, x32, w32, k, steps=8, tol=1e-11):
         e[close] = -0.5 * g[close]
         off = np.ones_like(e, dtype=bool)
         off[np.arange(k), np.arange(k)] = False
-        hist.append(float(np.max(np.abs(e[off]))))
+        off &= ~(lam_j >= 0.0)[None, :]     # only the columns |A+| takes
+        hist.append(float(np.max(np.abs(e[off]), initial=0.0)))
         e[np.arange(k), np.arange(k)] = 0.5 * (1.0 - np.diag(g[:k]))
         x[:, :k] = xj + x @ e
         if hist[-1] < tol:

@@ -99,8 +99,10 @@ def test_batch_form_mixed_within_1ueh(workload):
     r = host.ACECORR_BATCH(workload["systems"], workload["alphas"], workload["weights"], prec=host.GMC_MIXED)
     print(f"batch form mixed: {time.perf_counter() - t0:.2f} s", flush=True)
     de = np.abs(r["ecorr"] - workload["fp64"])
-    print(f"batch form mixed vs per-system FP64: max |dEcorr| {de.max():.2e}", flush=True)
-    assert np.all(de < 1e-6), (np.argmax(de), de.max())
+    k = int(np.argmax(de))
+    print(f"batch form mixed vs per-system FP64: max |dEcorr| {de.max():.2e} (system {k}; its per-system mixed "
+          f"{abs(workload['mixed'][k] - workload['fp64'][k]):.2e})", flush=True)
+    assert np.all(de < 1e-6), (k, de.max())
 
 
 def test_batch_form_graph_replays(workload):

@@ -33,6 +33,9 @@ DRIVER = textwrap.dedent(r"""
         std::printf("eff %g %g %g %g\n", oa_effective(1e-3f, 10, 4096), oa_effective(1e-3f, 4097, 4096),
                     oa_effective(NAN, 0, 4096), oa_effective(INFINITY, 0, 4096));
         std::printf("eff0 %g\n", oa_effective(2e-9f, 1, 0));
+        // oa_measure: the prediction counts 50-fold; non-finite parts read +inf
+        std::printf("meas %g %g %g %g\n", oa_measure(4e-5f, 1.8e-4f), oa_measure(2.8e-4f, 1e-6f),
+                    oa_measure(1e-3f, NAN), oa_measure(NAN, 0.f));
         // adaptive mode: n_min 3, n_max 6, re-solve available
         show("early_continue", {1e-3f, 1e-2f}, 0, 3, 6, true);
         show("early_stop", {1e-9f, 1e-10f}, 0, 3, 6, true);
@@ -71,6 +74,15 @@ def test_effective_correction(results):
     assert results["eff"] == ["0.001", "inf", "inf", "inf"]
     # a cap of 0 records no pair: any close pair makes the problem unconverged
     assert results["eff0"] == ["inf"]
+
+
+def test_measure_counts_the_predicted_error(results):
+    """A last correction of 4e-5 with a pair predicted at 1.8e-4 (a gap of
+    1e-5 of the spectrum) is not converged at the 5e-4 threshold; 2.8e-4
+    with well separated pairs is; non-finite parts are +inf."""
+    m = [float(x) for x in results["meas"]]
+    assert m[0] >= 5e-4 and m[1] < 5e-4 and m[1] == pytest.approx(2.8e-4, rel=1e-6)
+    assert m[2] == float("inf") and m[3] == float("inf")
 
 
 def test_adaptive_mode(results):
