@@ -1513,11 +1513,35 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         GMC_TRY(gemm_big<double>(st, gS, 1.0, X, X, 0.0, AX));
         hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, AX, n);
     }
-    // M = F^T A- F (into Ap) for the factor F in AX (L, or S = S^T)
-    GemmShape gN{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
-    GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
-    GemmShape gTl{n, n, n, true, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
-    GMC_TRY(gemm_big<double>(st, gTl, 1.0, AX, Sb, 0.0, Ap));   // M is symmetric: its lower triangle, mirrored
+    // M = F^T A- F (into Ap) for the factor F in AX (L, or S = S^T).  With L
+    // the products run over TRI_SLICES slices that skip its zero triangle:
+    // A- L by column slices (columns [c0, c1) need rows c0.. of L), L^T Sb
+    // by row slices of the lower triangle (rows [r0, r1) need rows r0.. of
+    // Sb and columns up to r1), L Z by row slices (rows [r0, r1) need
+    // columns up to r1 of L): 9/16, 15/64 and 9/16 of the full products.
+    const int64_t snn = (int64_t)nn;
+    constexpr int TRI_SLICES = 8;
+    auto cut = [&](int p) { return (int)((int64_t)n * p / TRI_SLICES); };
+    GemmShape gN{n, n, n, false, false, n, n, n, snn, snn, snn, B, 0};
+    if (chol_ok) {
+        for (int p = 0; p < TRI_SLICES; p++) {
+            const int c0 = cut(p), c1 = cut(p + 1);
+            if (c1 <= c0) continue;
+            GemmShape g{n, c1 - c0, n - c0, false, false, n, n, n, snn, snn, snn, B, 0};
+            GMC_TRY(gemm_big<double>(st, g, 1.0, Am + (size_t)c0 * n, AX + c0 + (size_t)c0 * n, 0.0,
+                                     Sb + (size_t)c0 * n));
+        }
+        for (int p = 0; p < TRI_SLICES; p++) {
+            const int r0 = cut(p), r1 = cut(p + 1);
+            if (r1 <= r0) continue;
+            GemmShape g{r1 - r0, r1, n - r0, true, false, n, n, n, snn, snn, snn, B, 0};
+            GMC_TRY(gemm_big<double>(st, g, 1.0, AX + r0 + (size_t)r0 * n, Sb + r0, 0.0, Ap + r0));
+        }
+    } else {
+        GMC_TRY(gemm_big<double>(st, gN, 1.0, Am, AX, 0.0, Sb));
+        GemmShape gTl{n, n, n, true, false, n, n, n, snn, snn, snn, B, 1};
+        GMC_TRY(gemm_big<double>(st, gTl, 1.0, AX, Sb, 0.0, Ap));   // M is symmetric: its lower triangle
+    }
     hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3(nt64, nt64, B), blk, 0, st, Ap, n);
     if (bt) hipLaunchKernelGGL(pad_diag_set_kernel, dim3(B), blk, 0, st, bt->sys, nalpha, Ap, n, -1.0);
     // eig(M) in FP32, refined in FP64 (Am is free now and serves as the AX buffer)
@@ -1545,7 +1569,16 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         fprintf(stderr, "\n");
     }
     // Y = F Z (Sb), filtered copy (Gb), P = Yf Yf^T (Am), integrand
-    GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
+    if (chol_ok) {
+        for (int p = 0; p < TRI_SLICES; p++) {
+            const int r0 = cut(p), r1 = cut(p + 1);
+            if (r1 <= r0) continue;
+            GemmShape g{r1 - r0, n, r1, false, false, n, n, n, snn, snn, snn, B, 0};
+            GMC_TRY(gemm_big<double>(st, g, 1.0, AX + r0, X, 0.0, Sb + r0));
+        }
+    } else {
+        GMC_TRY(gemm_big<double>(st, gN, 1.0, AX, X, 0.0, Sb));
+    }
     hipLaunchKernelGGL(ycols_kernel<double>, gsq, blk, 0, st, Sb, Gb, om2, out.eig, n);
     GemmShape gP{n, n, n, false, true, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 1};
     GMC_TRY(gemm_big<double>(st, gP, 1.0, Gb, Gb, 0.0, Am));

@@ -16,11 +16,14 @@ namespace gmc {
 namespace {
 
 constexpr int LDSW = CHOL_NB + 1;   // padded column of the LDS block
-constexpr size_t DIAG_LDS = ((size_t)CHOL_NB * LDSW + CHOL_NB) * sizeof(double);
+constexpr size_t DIAG_LDS = ((size_t)CHOL_NB * LDSW + 2 * CHOL_NB) * sizeof(double);
+static_assert(CHOL_NB == 128, "potrf_diag_kernel maps 256 threads to 128 rows");
 
 // Factor of the kb x kb diagonal block at (k, k), written back, and its
 // inverse into Linv.  a[c][r] holds block element (r, c).  A problem with
-// info != 0 is skipped.
+// info != 0 is skipped.  Threads own rows: thread t works on row t % 128 and
+// on the columns of parity t / 128, so no index is divided and a column's
+// entries are read by consecutive threads.
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k, int kb, double* Linv, int* info)
 {
     extern __shared__ double lds[];
@@ -28,12 +31,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
     double* tmp = lds + CHOL_NB * LDSW;
     __shared__ int bad;
     const int b = blockIdx.x, t = threadIdx.x;
+    const int r = t & (CHOL_NB - 1), half = t / CHOL_NB;   // blockDim.x == 2 * CHOL_NB
     if (info[b] != 0) return;
     double* Ab = A + (size_t)b * n * n;
-    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-        const int r = idx % kb, c = idx / kb;
-        a[c][r] = r >= c ? Ab[(size_t)(k + c) * n + k + r] : 0.0;
-    }
+    if (r < kb)
+        for (int c = half; c < kb; c += 2) a[c][r] = r >= c ? Ab[(size_t)(k + c) * n + k + r] : 0.0;
     if (t == 0) bad = 0;
     __syncthreads();
     for (int j = 0; j < kb; j++) {
@@ -44,12 +46,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
         }
         const double l = sqrt(p);
         __syncthreads();
-        for (int r = j + t; r < kb; r += blockDim.x) a[j][r] = r == j ? l : a[j][r] / l;
+        if (half == 0 && r >= j && r < kb) a[j][r] = r == j ? l : a[j][r] / l;
         __syncthreads();
-        const int m = kb - 1 - j;
-        for (int idx = t; idx < m * m; idx += blockDim.x) {
-            const int r = j + 1 + idx % m, c = j + 1 + idx / m;
-            if (r >= c) a[c][r] -= a[j][r] * a[j][c];
+        if (r > j && r < kb) {
+            const double lr = a[j][r];
+            for (int c = j + 1 + half; c <= r; c += 2) a[c][r] -= lr * a[j][c];
         }
         __syncthreads();
     }
@@ -58,30 +59,29 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
         if (t == 0) info[b] = bad;
         return;
     }
-    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-        const int r = idx % kb, c = idx / kb;
-        Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
-    }
+    if (r < kb)
+        for (int c = half; c < kb; c += 2) Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
     // in-place inverse, columns from the right: column j of L^-1 below the
-    // diagonal is -(1/l_jj) Linv[j+1:, j+1:] l[j+1:, j]
+    // diagonal is -(1/l_jj) Linv[j+1:, j+1:] l[j+1:, j]; the two threads of
+    // a row split the sum by parity and combine through tmp2
+    double* tmp2 = tmp + CHOL_NB;
     for (int j = kb - 1; j >= 0; j--) {
         __syncthreads();
         const double djj = 1.0 / a[j][j];
-        for (int i = j + 1 + t; i < kb; i += blockDim.x) tmp[i] = a[j][i];
+        if (half == 0 && r > j && r < kb) tmp[r] = a[j][r];
         __syncthreads();
-        for (int i = j + 1 + t; i < kb; i += blockDim.x) {
-            double s = 0.0;
-            for (int q = j + 1; q <= i; q++) s += a[q][i] * tmp[q];
-            a[j][i] = -djj * s;
-        }
+        double s = 0.0;
+        if (r > j && r < kb)
+            for (int q = j + 1 + half; q <= r; q += 2) s += a[q][r] * tmp[q];
+        if (half == 1) tmp2[r] = s;
+        __syncthreads();
+        if (half == 0 && r > j && r < kb) a[j][r] = -djj * (s + tmp2[r]);
         if (t == 0) a[j][j] = djj;
     }
     __syncthreads();
     double* Lb = Linv + (size_t)b * CHOL_NB * CHOL_NB;
-    for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-        const int r = idx % kb, c = idx / kb;
-        Lb[(size_t)c * CHOL_NB + r] = r >= c ? a[c][r] : 0.0;
-    }
+    if (r < kb)
+        for (int c = half; c < kb; c += 2) Lb[(size_t)c * CHOL_NB + r] = r >= c ? a[c][r] : 0.0;
 }
 
 // P = A21: rows k + kb .. n-1, columns k .. k + kb - 1 (P: ld n, stride n CHOL_NB).
