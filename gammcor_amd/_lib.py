@@ -57,6 +57,7 @@ SIGNATURES = {
     "gmc_acecorr": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int,
                     _vp, _vp, _vp, _vp, _vp, _vp],
     "gmc_forget_resident": [],
+    "gmc_set_deterministic": [_c_int],
     "gmc_gemm_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
                      _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
 }

@@ -289,6 +289,12 @@ int gmc_twono1(double* tno, const double* ure, int nbasis, int64_t ninte2)
     return 0;
 }
 
+int gmc_set_deterministic(int on)
+{
+    gmc::set_deterministic(on != 0);
+    return 0;
+}
+
 int gmc_forget_resident(void)
 {
     State& st = state();

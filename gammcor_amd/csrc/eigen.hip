@@ -1,6 +1,8 @@
 // Batched symmetric eigensolver (see eigen.hpp for the structure).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 
 #include "eigen.hpp"
@@ -92,6 +94,46 @@ __device__ inline void group_barrier(unsigned* flags, int gi, int G, unsigned& e
     __syncthreads();
 }
 
+// ||A_b||_F of each column-major n x n problem (both triangles, as the
+// matrix arrives), one workgroup of 256 threads per problem: every thread
+// sums a fixed set of elements in a fixed order, the workgroup sum follows a
+// fixed tree, so the result has the same bits on every call.
+template <class T>
+__global__ void frob_kernel(const T* A, int n, T* out)
+{
+    __shared__ double part[256];
+    const T* a = A + (size_t)blockIdx.x * n * n;
+    const int64_t nn = (int64_t)n * n;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < nn; i += 256) acc += (double)a[i] * (double)a[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = (T)sqrt(part[0]);
+}
+
+namespace {
+std::atomic<int> g_det{-1};   // -1: not read yet from GMC_DETERMINISTIC
+}
+
+bool deterministic()
+{
+    int v = g_det.load();
+    if (v < 0) {
+        const char* e = getenv("GMC_DETERMINISTIC");
+        v = (e && atoi(e) != 0) ? 1 : 0;
+        int expect = -1;
+        g_det.compare_exchange_strong(expect, v);
+        v = g_det.load();
+    }
+    return v != 0;
+}
+
+void set_deterministic(bool on) { g_det.store(on ? 1 : 0); }
+
 __global__ void zero_u32(unsigned* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,6 +148,7 @@ struct PanelArgs {
     T* vg;   // B x G x n: per-workgroup copy of v when it does not fit in LDS
     unsigned* bar;
     int* err;
+    const T* fro;   // deterministic mode: Frobenius norm of each problem's matrix
     int n, k, kb, nb, G;
     unsigned epoch0;
     int vec;   // 16-byte SYMV path: n a multiple of 16 / sizeof(T) and A 16-byte aligned
@@ -144,6 +187,38 @@ constexpr int SY_THREADS = 1024, SY_WAVES = SY_THREADS / 64;
 __device__ inline void lds_add(double* a, double v) { __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void lds_add(float* a, float v) { __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
+// Deterministic mode: the SYMV's scatter sums arrive in any order, so they
+// are accumulated as fixed-point integers (an integer sum has the same bits
+// in any order).  Every partial sum of |A_rc v_c| is at most ||A||_F ||v||_2
+// (Cauchy-Schwarz on the trailing matrix, an orthogonal transform of the
+// start), so with the scale s = 2^29 / (||A||_F ||v||_2) (FP32, int32) or
+// 2^60 / (...) (FP64, int64) no sum overflows, and each term is rounded to
+// 2^-29 (2^-60) of that bound: below FP32's (FP64's) own rounding of the
+// floating-point sum.  The integers share the float slot of the same width.
+__device__ inline void det_add(float* a, float v, float sc)
+{
+    __hip_atomic_fetch_add(reinterpret_cast<int*>(a), __float2int_rn(v * sc), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void det_add(double* a, double v, double sc)
+{
+    __hip_atomic_fetch_add(reinterpret_cast<long long*>(a), __double2ll_rn(v * sc), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <bool DET, class T>
+__device__ inline void scat(T* a, T v, T sc)
+{
+    if constexpr (DET) det_add(a, v, sc);
+    else lds_add(a, v);
+}
+__device__ inline float det_value(const float* a, float sc) { return (float)*reinterpret_cast<const int*>(a) / sc; }
+__device__ inline double det_value(const double* a, double sc)
+{
+    return (double)*reinterpret_cast<const long long*>(a) / sc;
+}
+template <class T>
+constexpr T det_range() { return sizeof(T) == 8 ? T(1152921504606846976.0) : T(536870912.0); }   // 2^60, 2^29
+
 // VLDS: keep v in LDS (n <= a few thousand); otherwise in a per-workgroup
 // global copy.  The SYMV reads only the lower triangle of the trailing
 // matrix: column r gives the dot sum_{c>=r} A[c,r] v[c] for y_r and scatters
@@ -167,7 +242,7 @@ __device__ inline Vec16<T> bld16(__amdgpu_buffer_rsrc_t r, int voff, int soff)
     return __builtin_bit_cast(Vec16<T>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-template <class T, bool VLDS, bool PLDS, int KV, int UV>
+template <class T, bool VLDS, bool PLDS, int KV, int UV, bool DET>
 __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 {
     // columns per wave in the SYMV: FP32 halves the HBM time per element, so
@@ -213,7 +288,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T* a2 = Wr + nb;
     T* t12 = a2 + nb;                               // 2 nb
     T* red = t12 + 2 * nb;                          // SY_WAVES
-    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal
+    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal [2]=fixed-point scale (DET)
 
     // panel entries of an owned row (slot q, row r): LDS copies or the global panel
     auto vpan = [&](int c, int q, int r) -> T {
@@ -285,6 +360,11 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                 }
                 sc[0] = tau;
                 sc[1] = scal;
+                if constexpr (DET) {
+                    // ||v||_2 with v(j+1) = 1 and the rest scal * x
+                    const T bound = g.fro[p] * sqrt(T(1) + scal * scal * xn2);
+                    sc[2] = bound > T(0) && bound < T(INFINITY) ? det_range<T>() / bound : T(1);
+                }
                 if (gi == 0) {
                     g.tau[(size_t)p * n + j] = tau;
                     g.e[(size_t)p * n + j] = beta;
@@ -293,6 +373,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             __syncthreads();
         }
         const T tau = sc[0], scal = sc[1];
+        const T dsc = DET ? sc[2] : T(0);
         for (int c = j + 1 + tid; c < n; c += SY_THREADS) {
             v[c] = c == j + 1 ? T(1) : scal * v[c];
             u[c] = T(0);
@@ -346,7 +427,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                         }
 #pragma unroll
                         for (int e = 0; e < VN; e++)
-                            if (c + e > j && c + e > r0) lds_add(&u[c + e], w[e]);
+                            if (c + e > j && c + e > r0) scat<DET>(&u[c + e], w[e], dsc);
                     }
                 }
                 // rows below the diagonal block through one buffer descriptor
@@ -381,7 +462,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                                 w[e] += a[uu][kk].x[e] * vk[kk];
                             }
 #pragma unroll
-                        for (int e = 0; e < VN; e++) lds_add(&u[c + uu * 64 * VN + e], w[e]);
+                        for (int e = 0; e < VN; e++) scat<DET>(&u[c + uu * 64 * VN + e], w[e], dsc);
                     }
                 }
                 for (; c < n; c += 64 * VN) {
@@ -399,7 +480,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                         }
                     }
 #pragma unroll
-                    for (int e = 0; e < VN; e++) lds_add(&u[c + e], w[e]);
+                    for (int e = 0; e < VN; e++) scat<DET>(&u[c + e], w[e], dsc);
                 }
 #pragma unroll
                 for (int kk = 0; kk < KV; kk++) {
@@ -440,15 +521,15 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                     s1 += x1 * v[c + 64];
                     s2 += x2 * v[c + 128];
                     s3 += x3 * v[c + 192];
-                    if (c > r0) lds_add(&u[c], x0 * vr);
-                    lds_add(&u[c + 64], x1 * vr);
-                    lds_add(&u[c + 128], x2 * vr);
-                    lds_add(&u[c + 192], x3 * vr);
+                    if (c > r0) scat<DET>(&u[c], x0 * vr, dsc);
+                    scat<DET>(&u[c + 64], x1 * vr, dsc);
+                    scat<DET>(&u[c + 128], x2 * vr, dsc);
+                    scat<DET>(&u[c + 192], x3 * vr, dsc);
                 }
                 for (; c < n; c += 64) {
                     const T x0 = c0p[c];
                     s[0] += x0 * v[c];
-                    if (c > r0) lds_add(&u[c], x0 * vr);
+                    if (c > r0) scat<DET>(&u[c], x0 * vr, dsc);
                 }
                 s[0] += (s1 + s2) + s3;
             } else {
@@ -463,7 +544,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                             if (c > r0 + kk) w += a * vk[kk];
                         }
                     }
-                    if (c > j && c > r0) lds_add(&u[c], w);
+                    if (c > j && c > r0) scat<DET>(&u[c], w, dsc);
                 }
             }
 #pragma unroll
@@ -473,6 +554,11 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             }
         }
         __syncthreads();
+        if constexpr (DET) {
+            // the fixed-point scatter sums back to T
+            for (int c = j + 1 + tid; c < n; c += SY_THREADS) u[c] = det_value(&u[c], dsc);
+            __syncthreads();
+        }
         {
             // v^T y partial of this workgroup: own rows' dot parts and its scatter partial
             T vy = 0;
@@ -513,7 +599,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             const T ydot1 = *p_ydot;
             if (i > 0) {
                 const int nc = 2 * i;
-                const int PP = max(1, min(G, SY_THREADS / nc));
+                const int PP = DET ? 1 : max(1, min(G, SY_THREADS / nc));   // DET: one summing thread
                 for (int idx = tid; idx < nc * PP; idx += SY_THREADS) {
                     const int c = idx % nc, part = idx / nc;
                     T acc = 0;
@@ -547,7 +633,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         const int qb = first_slot(j + 1, gi, G), nbq = nslots - qb;
         {
             const int nterm = G + i;
-            const int P = nbq > 0 ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
+            const int P = (nbq > 0 && !DET) ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
             for (int idx = tid; idx < nbq * P; idx += SY_THREADS) {
                 const int q = qb + idx % nbq, part = idx / nbq;
                 const int r = owned_row(q, gi, G);
@@ -589,7 +675,7 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
         // each row's i+1 terms split over P threads into u
         {
             const int nterm = i + 1;
-            const int P = nbq > 0 ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
+            const int P = (nbq > 0 && !DET) ? max(1, min(nterm, SY_THREADS / nbq)) : 1;
             for (int idx = tid; idx < nbq * P; idx += SY_THREADS) {
                 const int qq = idx % nbq, part = idx / nbq;
                 const int q = qb + qq;
@@ -1236,6 +1322,7 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->Tf, B * 2 * nbb * nbb * sizeof(T));
     take(p->bar, (B * G + 1) * BAR_STRIDE * sizeof(unsigned));
     take(p->err, 64);
+    take(p->fro, B * sizeof(T));
     p->ws_bytes = align_up(off);
     return 0;
 }
@@ -1449,10 +1536,21 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         // 16-byte SYMV shape per type (measured at n=2664/4524, batch 16):
         // FP32 8 columns x 1 chunk, FP64 4 columns x 2 chunks in flight
         constexpr int KV = sizeof(T) == 8 ? 4 : 8, UV = sizeof(T) == 8 ? 2 : 1;
-        const void* kfn = vlds ? (plds ? (const void*)sytrd_panel_kernel<T, true, true, KV, UV>
-                                       : (const void*)sytrd_panel_kernel<T, true, false, KV, UV>)
-                               : (plds ? (const void*)sytrd_panel_kernel<T, false, true, KV, UV>
-                                       : (const void*)sytrd_panel_kernel<T, false, false, KV, UV>);
+        const bool det = deterministic();
+        const void* kfn =
+            det ? (vlds ? (plds ? (const void*)sytrd_panel_kernel<T, true, true, KV, UV, true>
+                                : (const void*)sytrd_panel_kernel<T, true, false, KV, UV, true>)
+                        : (plds ? (const void*)sytrd_panel_kernel<T, false, true, KV, UV, true>
+                                : (const void*)sytrd_panel_kernel<T, false, false, KV, UV, true>))
+                : (vlds ? (plds ? (const void*)sytrd_panel_kernel<T, true, true, KV, UV, false>
+                                : (const void*)sytrd_panel_kernel<T, true, false, KV, UV, false>)
+                        : (plds ? (const void*)sytrd_panel_kernel<T, false, true, KV, UV, false>
+                                : (const void*)sytrd_panel_kernel<T, false, false, KV, UV, false>));
+        if (det) {
+            // the Frobenius norm of each problem, summed in a fixed order
+            hipLaunchKernelGGL(frob_kernel<T>, dim3(B), dim3(256), 0, s, A, n, p.fro);
+            GMC_HIP(hipGetLastError());
+        }
         // (set by an eager call before a capture; not repeated inside one)
         if (!capturing(s)) GMC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         unsigned epoch = 0;
@@ -1460,8 +1558,8 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             const int kb = std::min(nb, n - 1 - k);
             const int vec = (n % (16 / (int)sizeof(T)) == 0) && (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
                             (reinterpret_cast<uintptr_t>(p.vg) % 16 == 0);
-            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, n, k, kb, nb, G, epoch,
-                           vec};
+            PanelArgs<T> a{A, p.W, p.X, p.d, p.e, p.tau, p.part, p.Pu, p.vg, p.bar, p.err, p.fro, n, k, kb, nb, G,
+                           epoch, vec};
             void* args[] = {&a};
             double bytes = 0.0;   // lower triangle of the trailing matrix, once per column
             for (int jj = k; jj < k + kb; jj++) bytes += 0.5 * (double)(n - jj - 1) * (double)(n - jj) * sizeof(T);

@@ -39,6 +39,7 @@ struct EigPlan {
     int *rotp = nullptr, *rotj = nullptr, *kcnt = nullptr, *rcnt = nullptr;
     unsigned* bar = nullptr;
     int* err = nullptr;
+    T* fro = nullptr;   // deterministic mode: ||A_b||_F of each problem
     // the barrier time-out flag holds until read: the first solve after a
     // bind clears it, later ones add to it (eig_check_error reads and clears)
     bool clear_err = true;
@@ -50,6 +51,13 @@ struct EigPlan {
     EigPlan& operator=(const EigPlan&) = delete;
     ~EigPlan();
 };
+
+// Deterministic mode of the eigensolver (GMC_DETERMINISTIC=1 or
+// gmc_set_deterministic): the panel kernel's sums that arrive in any order
+// are accumulated in fixed point or by one thread, so repeated solves of the
+// same matrices give the same bits.
+bool deterministic();
+void set_deterministic(bool on);
 
 template <class T>
 int eig_plan(int n, int batch, EigPlan<T>* p);

@@ -138,6 +138,15 @@ int gmc_acecorr(int prec, int alpha_batch, int nbasis, int ninact, int nact, con
  * counterpart; for callers that modify TwoNO in place between calls). */
 int gmc_forget_resident(void);
 
+/* Deterministic mode on (1) or off (0), for the calls that follow (no
+ * reference counterpart; the reference's MKL path is not reproducible bit
+ * for bit across thread counts either).  On, the eigensolver's sums whose
+ * terms arrive in any order are accumulated in fixed point or by a single
+ * thread, so repeating a solve of the same inputs gives the same bits; the
+ * tridiagonalisation runs somewhat slower.  GMC_DETERMINISTIC=1 in the
+ * environment sets it at load. */
+int gmc_set_deterministic(int on);
+
 /* Building block exported for tests: batched column-major GEMM on MFMA,
  * C_b = alpha op(A_b) op(B_b) + beta C_b.  prec = GMC_FP64 or GMC_FP32 (then
  * the pointers are float and alpha/beta are rounded to float).  No
