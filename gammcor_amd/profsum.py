@@ -7,6 +7,8 @@
                                                         # calibrated bytes/launch per timing name
     python -m gammcor_amd.profsum copy [--mib M] [--reps R]
                                                         # device copy of known size (calibration run)
+    python -m gammcor_amd.profsum mfma <dir>             # --pmc MfmaUtil MfmaFlopsF64 MfmaFlopsF32:
+                                                        # per kernel MFMA flops, TFLOP/s and utilisation
 
 `<dir>` is the output directory of a `rocprofv3 -f csv -d <dir> -o run ...`
 run (the CSV files are found below it).  For `pmc`, a run that starts with
@@ -133,6 +135,9 @@ def main(argv=None):
     t.add_argument("write_dir")
     t.add_argument("--calib-first", type=int, default=3)
     t.add_argument("--calib-mib", type=int, default=1024)
+    m = sub.add_parser("mfma")
+    m.add_argument("dir")
+    m.add_argument("--top", type=int, default=25)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
     c.add_argument("--reps", type=int, default=5)
@@ -140,6 +145,32 @@ def main(argv=None):
 
     if args.cmd == "copy":
         copy_run(args.mib, args.reps)
+        return
+    if args.cmd == "mfma":
+        # per dispatch: MfmaUtil (% of the SIMDs' cycles with the MFMA unit
+        # busy), MfmaFlopsF64 / MfmaFlopsF32 (flops issued), and the
+        # dispatch's start/end stamps; aggregated per kernel name
+        path = _find(args.dir, "*counter_collection.csv")
+        disp = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                d = disp.setdefault(int(_col(row, "Dispatch_Id")), {"k": _col(row, "Kernel_Name")})
+                d[_col(row, "Counter_Name")] = float(_col(row, "Counter_Value"))
+                d["t"] = (int(_col(row, "End_Timestamp")) - int(_col(row, "Start_Timestamp"))) * 1e-9
+        per = collections.defaultdict(lambda: [0, 0.0, 0.0, 0.0, 0.0])   # n, seconds, f64, f32, util*seconds
+        for d in disp.values():
+            a = per[d["k"]]
+            a[0] += 1
+            a[1] += d["t"]
+            a[2] += d.get("MfmaFlopsF64", 0.0)
+            a[3] += d.get("MfmaFlopsF32", 0.0)
+            a[4] += d.get("MfmaUtil", 0.0) * d["t"]
+        rows = sorted(((k, *v) for k, v in per.items() if v[2] + v[3] > 0), key=lambda r: -r[2])
+        print(f"# source: {os.path.relpath(path, args.dir)}; kernels that issued FP64 or FP32 MFMA")
+        print(f"{'calls':>6} {'ms':>9} {'f64_TF':>8} {'f32_TF':>8} {'TF/s':>8} {'util%':>6}  kernel")
+        for k, n, t, f64, f32, ut in rows[: args.top]:
+            print(f"{n:6d} {1e3 * t:9.1f} {f64 * 1e-12:8.2f} {f32 * 1e-12:8.2f} {(f64 + f32) * 1e-12 / t:8.1f} "
+                  f"{ut / t:6.1f}  {_short(k)}")
         return
     if args.cmd == "traffic":
         # calibrated HBM bytes per launch, FETCH_SIZE + WRITE_SIZE, per timing name
