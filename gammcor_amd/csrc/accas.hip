@@ -863,39 +863,8 @@ __global__ void oa_delta_kernel(const double* partial, int np, double* delta, in
 // pairs than recorded is treated as not converged (oa_policy.hpp).
 constexpr int OA_MAXPAIRS = 4096;
 constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
+constexpr int OA_ET = 32;           // tile of the E kernel
 constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
-
-__device__ float2 oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
-                           unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j, int wmode);
-
-__global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
-                                                   const double* delta, double* E, int n, unsigned* npair, int4* pairs,
-                                                   double2* rot, unsigned cap, unsigned* emax, unsigned* pmax,
-                                                   int wmode)
-{
-    const int b = blockIdx.z;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    // largest off-diagonal correction and prediction of the block, one
-    // atomic each per block (oa_e_one returns +inf for a NaN element, so
-    // fmaxf keeps it)
-    __shared__ float wmax[4], wpred[4];
-    float2 my = make_float2(0.f, 0.f);
-    if (i < n) my = oa_e_one(S, G, lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode);
-    for (int o = 32; o > 0; o >>= 1) {
-        my.x = fmaxf(my.x, __shfl_xor(my.x, o, 64));
-        my.y = fmaxf(my.y, __shfl_xor(my.y, o, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        wmax[threadIdx.x >> 6] = my.x;
-        wpred[threadIdx.x >> 6] = my.y;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const size_t slot = (size_t)b * OA_EMAX_SLOTS + (blockIdx.y % OA_EMAX_SLOTS);
-        atomicMax(&emax[slot], __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
-        atomicMax(&pmax[slot], __float_as_uint(fmaxf(fmaxf(wpred[0], wpred[1]), fmaxf(wpred[2], wpred[3]))));
-    }
-}
 
 // The weight of eigenvalue l in what the solve makes of the eigenvectors:
 // wmode 1, sqrt|l| (S = V |L|^(1/2) V^T of A+); wmode 2, the ERPA integrand's
@@ -909,7 +878,8 @@ __device__ inline double oa_weight(double l, int wmode)
     return om > 1e-3 && om < 1e8 ? 0.5 / om : 0.0;
 }
 
-// One element of E; returns (|E_ij|, the predicted next correction of the
+// One element of E from s_ij and g_ij (the lower-triangle values of S and
+// G = X^T X); returns (|E_ij|, the predicted next correction of the
 // pair) for i != j ((0, 0) on the diagonal), +inf when not finite.  The
 // separated-or-close test is symmetric in (i, j) (the larger of
 // |s_ij + l_j r_ij| and |s_ij + l_i r_ij|), so E_ij and E_ji always take the
@@ -922,14 +892,12 @@ __device__ inline double oa_weight(double l, int wmode)
 // or two weights of 0): large for an ERPA mode the filter keeps next to one
 // it drops (a gap of 1e-5 of the spectrum in one system of the 64-system
 // workload), small for two kept modes of nearly equal omega.
-__device__ float2 oa_e_one(const double* S, const double* G, const double* lam, const double* delta, double* E, int n,
+__device__ float2 oa_e_one(double sij, double gij, const double* lam, const double* delta, double* E, int n,
                            unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j, int wmode)
 {
     const size_t base = (size_t)b * n * n;
     const int hi = i > j ? i : j, lo = i > j ? j : i;
-    const size_t at = base + hi + (size_t)lo * n;
-    const double sij = S[at];
-    const double rij = (i == j ? 1.0 : 0.0) - G[at];
+    const double rij = (i == j ? 1.0 : 0.0) - gij;
     const double li = lam[(size_t)b * n + i], lj = lam[(size_t)b * n + j];
     const double num = sij + lj * rij, den = lj - li;
     const double nsym = fmax(fabs(num), fabs(sij + li * rij));
@@ -963,6 +931,67 @@ __device__ float2 oa_e_one(const double* S, const double* G, const double* lam, 
     if (i == j) return make_float2(0.f, 0.f);
     const float r = (float)fabs(e), q = (float)pred;
     return make_float2(r <= 3.4028235e38f ? r : INFINITY, q <= 3.4028235e38f ? q : INFINITY);
+}
+
+// E of one problem by OA_ET x OA_ET tile pairs (I >= J): S and G are valid
+// on and below the diagonal, so the lower tile (rows of block I, columns of
+// block J) is read once, along its columns, into LDS and serves both E(I, J)
+// and its mirror E(J, I) (E is not symmetric); E is written along columns
+// too.  One workgroup per tile pair and problem (blockIdx.y).
+__global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
+                                                   const double* delta, double* E, int n, unsigned* npair, int4* pairs,
+                                                   double2* rot, unsigned cap, unsigned* emax, unsigned* pmax,
+                                                   int wmode)
+{
+    __shared__ double st[OA_ET][OA_ET + 1], gt[OA_ET][OA_ET + 1];
+    __shared__ float wmax[4], wpred[4];
+    const int b = blockIdx.y, t = blockIdx.x;
+    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    while (I * (I + 1) / 2 > t) I--;
+    const int J = t - I * (I + 1) / 2;
+    const int r0 = OA_ET * I, c0 = OA_ET * J;
+    const size_t base = (size_t)b * n * n;
+    for (int idx = threadIdx.x; idx < OA_ET * OA_ET; idx += 256) {
+        const int r = idx % OA_ET, c = idx / OA_ET;
+        const int gi = r0 + r, gj = c0 + c;
+        const bool ok = gi < n && gj < n && gi >= gj;
+        st[c][r] = ok ? S[base + gi + (size_t)gj * n] : 0.0;
+        gt[c][r] = ok ? G[base + gi + (size_t)gj * n] : 0.0;
+    }
+    __syncthreads();
+    // (oa_e_one returns +inf for a NaN element, so fmaxf keeps it)
+    float2 my = make_float2(0.f, 0.f);
+    auto take = [&](float2 v) {
+        my.x = fmaxf(my.x, v.x);
+        my.y = fmaxf(my.y, v.y);
+    };
+    for (int idx = threadIdx.x; idx < OA_ET * OA_ET; idx += 256) {
+        const int r = idx % OA_ET, c = idx / OA_ET;
+        // the lower tile: (i, j) = (r0 + r, c0 + c), i >= j
+        int i = r0 + r, j = c0 + c;
+        if (i < n && j < n && i >= j)
+            take(oa_e_one(st[c][r], gt[c][r], lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode));
+        // its mirror: (i, j) = (c0 + r, r0 + c), i < j, from the lower element (j, i)
+        i = c0 + r;
+        j = r0 + c;
+        if (i < n && j < n && i < j)
+            take(oa_e_one(st[r][c], gt[r][c], lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        my.x = fmaxf(my.x, __shfl_xor(my.x, o, 64));
+        my.y = fmaxf(my.y, __shfl_xor(my.y, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = my.x;
+        wpred[threadIdx.x >> 6] = my.y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const size_t slot = (size_t)b * OA_EMAX_SLOTS + (t % OA_EMAX_SLOTS);
+        atomicMax(&emax[slot], __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+        atomicMax(&pmax[slot], __float_as_uint(fmaxf(fmaxf(wpred[0], wpred[1]), fmaxf(wpred[2], wpred[3]))));
+    }
 }
 
 // x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
@@ -1072,8 +1101,9 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
     GMC_HIP(hipMemsetAsync(op.npair, 0, B * sizeof(unsigned), st));
     GMC_HIP(hipMemsetAsync(op.emax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     GMC_HIP(hipMemsetAsync(op.pmax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
-    hipLaunchKernelGGL(oa_e_kernel, dim3((n + 255) / 256, n, B), dim3(256), 0, st, S, G, lam, delta, AX, n, op.npair,
-                       op.pairs, op.rot, op.cap, op.emax, op.pmax, wmode);
+    const int nte = (n + OA_ET - 1) / OA_ET;
+    hipLaunchKernelGGL(oa_e_kernel, dim3(nte * (nte + 1) / 2, B), dim3(256), 0, st, S, G, lam, delta, AX, n,
+                       op.npair, op.pairs, op.rot, op.cap, op.emax, op.pmax, wmode);
     GMC_HIP(hipGetLastError());
     // each problem's largest off-diagonal correction and prediction
     // (non-negative floats order as their bits; +inf marks a non-finite

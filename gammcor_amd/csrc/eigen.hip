@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <functional>
+#include <limits>
 
 #include "eigen.hpp"
 #include "gemm_tile.hpp"
@@ -1275,6 +1276,65 @@ __global__ __launch_bounds__(256) void dc_final2_kernel(DcArgs a, DcBufs<T> b)
     for (int r = threadIdx.x; r < m; r += 256) dst[r] = src[r];
 }
 
+// Padded problems (EigPlan::np > n): Ap = diag(A, 0) of order np.  The
+// Householder vectors of the reduction then vanish on the pad rows, so the
+// pad rows and columns stay zero apart from their diagonal and the
+// subdiagonal e is exactly zero from row n - 1 on.
+template <class T>
+__global__ void pad_in_kernel(const T* A, int n, T* Ap, int np)
+{
+    const int b = blockIdx.y;
+    const T* a = A + (size_t)b * n * n;
+    T* ap = Ap + (size_t)b * np * np;
+    const int64_t total = (int64_t)np * np;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % np), c = (int)(idx / np);
+        ap[idx] = (r < n && c < n) ? a[r + (size_t)c * n] : T(0);
+    }
+}
+
+// The pad diagonal of the tridiagonal matrix: c = max_i (d_i + |e_i-1| + |e_i|)
+// over the n rows of the problem (Gershgorin: no eigenvalue of A lies above
+// it) plus 2^-10 of the largest Gershgorin radius, so the np - n pad
+// eigenvalues come last in ascending order and never deflate against an
+// eigenvalue of A.  One workgroup per problem.
+template <class T>
+__global__ __launch_bounds__(256) void pad_tail_kernel(T* d, T* e, int n, int np)
+{
+    __shared__ T red[8];
+    T* dd = d + (size_t)blockIdx.x * np;
+    T* ee = e + (size_t)blockIdx.x * np;
+    T hi = -INFINITY, mag = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const T off = (i > 0 ? fabs(ee[i - 1]) : T(0)) + (i < n - 1 ? fabs(ee[i]) : T(0));
+        hi = fmax(hi, dd[i] + off);
+        mag = fmax(mag, fabs(dd[i]) + off);
+    }
+    hi = block_max(hi, red);
+    mag = block_max(mag, red);
+    const T c = hi + T(0.0009765625) * mag + std::numeric_limits<T>::min();
+    for (int i = n + threadIdx.x; i < np; i += 256) dd[i] = c;
+    for (int i = n - 1 + threadIdx.x; i < np - 1; i += 256) ee[i] = T(0);
+}
+
+// Eigenvectors and eigenvalues of the padded problems back to order n: the
+// first n eigenpairs are those of A, and their pad rows are zero.
+template <class T>
+__global__ void pad_out_kernel(const T* Z, const T* wp, int np, T* A, T* w, int n)
+{
+    const int b = blockIdx.y;
+    const T* z = Z + (size_t)b * np * np;
+    T* a = A + (size_t)b * n * n;
+    const int64_t total = (int64_t)n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % n), c = (int)(idx / n);
+        a[idx] = z[r + (size_t)c * np];
+        if (r == 0) w[(size_t)b * n + c] = wp[(size_t)b * np + c];
+    }
+}
+
 template <class T>
 __global__ void copy_kernel(const T* src, T* dst, int64_t count)
 {
@@ -1296,7 +1356,7 @@ EigPlan<T>::~EigPlan()
 template <class T>
 static int eig_layout(EigPlan<T>* p, char* base)
 {
-    const size_t B = p->batch, n = p->n, nn = n * n, nb = p->nb, nbb = p->nbb, G = p->G;
+    const size_t B = p->batch, n = p->np, nn = n * n, nb = p->nb, nbb = p->nbb, G = p->G;
     size_t off = 0;
     auto take = [&](auto*& ptr, size_t bytes) {
         size_t o = align_up(off);
@@ -1323,6 +1383,12 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->bar, (B * G + 1) * BAR_STRIDE * sizeof(unsigned));
     take(p->err, 64);
     take(p->fro, B * sizeof(T));
+    if (p->np != p->n) {
+        take(p->Ap, B * nn * sizeof(T));
+        take(p->wp, B * n * sizeof(T));
+    } else {
+        p->Ap = p->wp = nullptr;
+    }
     p->ws_bytes = align_up(off);
     return 0;
 }
@@ -1336,6 +1402,11 @@ static int eig_plan_host(int n, int batch, EigPlan<T>* p, std::vector<std::vecto
     GMC_CHECK(n >= 1 && batch >= 1, "eig_plan: bad sizes n=%d batch=%d", n, batch);
     p->n = n;
     p->batch = batch;
+    {
+        constexpr int vw = 16 / (int)sizeof(T);
+        p->np = (n >= EIG_PAD_MIN && n % vw != 0) ? (n + vw - 1) / vw * vw : n;
+    }
+    n = p->np;   // the tree, G and the workspace are those of the padded order
     // workgroups per problem: all problems resident at once (one per CU)
     int ncu = 256;
     {
@@ -1506,8 +1577,17 @@ template <class T>
 int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
 {
     GMC_CHECK(p.bound, "syevd: plan not bound to a workspace");
-    const int n = p.n, B = p.batch, nb = p.nb, G = p.G;
+    const int n = p.np, B = p.batch, nb = p.nb, G = p.G;
     const size_t nn = (size_t)n * n;
+    const bool padded = p.np != p.n;
+    T* const Au = A;
+    T* const wu = w;
+    if (padded) {
+        hipLaunchKernelGGL(pad_in_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, Au, p.n, p.Ap, n);
+        GMC_HIP(hipGetLastError());
+        A = p.Ap;
+        w = p.wp;
+    }
     hipLaunchKernelGGL(zero_u32, dim3(((B * G + 1) * BAR_STRIDE + 255) / 256), dim3(256), 0, s, p.bar,
                        (B * G + 1) * BAR_STRIDE);
     if (p.clear_err) {
@@ -1579,6 +1659,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     }
     hipLaunchKernelGGL(sytrd_finish_kernel<T>, dim3((B + 255) / 256), dim3(256), 0, s, A, p.d, n, B);
     hipLaunchKernelGGL(clear_upper_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, A, n);
+    if (padded) hipLaunchKernelGGL(pad_tail_kernel<T>, dim3(B), dim3(256), 0, s, p.d, p.e, p.n, n);
     GMC_HIP(hipGetLastError());
     ps_tri.close();
     ProfScope ps_dc(s, f64 ? "eig_dc_f64" : "eig_dc_f32", 0.0);
@@ -1642,7 +1723,11 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
             else GMC_TRY(gemm_big<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
         }
     }
-    hipLaunchKernelGGL(copy_kernel<T>, dim3(grid1((int64_t)B * nn)), dim3(256), 0, s, p.Q, A, (int64_t)B * nn);
+    if (padded)
+        hipLaunchKernelGGL(pad_out_kernel<T>, dim3(grid1((int64_t)p.n * p.n), B), dim3(256), 0, s, p.Q, p.wp, n, Au,
+                           wu, p.n);
+    else
+        hipLaunchKernelGGL(copy_kernel<T>, dim3(grid1((int64_t)B * nn)), dim3(256), 0, s, p.Q, A, (int64_t)B * nn);
     GMC_HIP(hipGetLastError());
     return 0;
 }

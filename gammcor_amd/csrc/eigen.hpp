@@ -5,7 +5,8 @@
 //   equation, Gu-Eisenstat z-hat, grouped MFMA merge GEMMs),
 //   blocked back transformation (MFMA GEMMs).
 // All B problems of one call share the order n; matrices are column-major
-// with leading dimension n and stride n*n.
+// with leading dimension n and stride n*n.  Orders that are not a multiple of
+// the 16-byte vector width are solved as a padded problem (EigPlan::np).
 #pragma once
 #include <vector>
 
@@ -23,6 +24,10 @@ struct DcLevel {
 template <class T>
 struct EigPlan {
     int n = 0, batch = 0;
+    // internal order: n, or n rounded up to the SYMV's 16-byte vector width
+    // (n >= EIG_PAD_MIN) so that every column starts 16-byte aligned; the
+    // padded problem diag(A, c I) has c above the spectrum of A
+    int np = 0;
     int nb = 32;            // tridiagonalisation panel width
     int nbb = 128;          // back-transformation block width (<= 128)
     int G = 1;              // workgroups cooperating on one problem
@@ -40,6 +45,7 @@ struct EigPlan {
     unsigned* bar = nullptr;
     int* err = nullptr;
     T* fro = nullptr;   // deterministic mode: ||A_b||_F of each problem
+    T *Ap = nullptr, *wp = nullptr;   // the padded problems and their eigenvalues (np != n)
     // the barrier time-out flag holds until read: the first solve after a
     // bind clears it, later ones add to it (eig_check_error reads and clears)
     bool clear_err = true;
@@ -58,6 +64,9 @@ struct EigPlan {
 // same matrices give the same bits.
 bool deterministic();
 void set_deterministic(bool on);
+
+// Smallest order that is padded when it is not a multiple of the vector width.
+constexpr int EIG_PAD_MIN = 256;
 
 template <class T>
 int eig_plan(int n, int batch, EigPlan<T>* p);

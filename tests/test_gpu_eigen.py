@@ -37,7 +37,10 @@ def _mats(kind, n, batch, seed):
                                           ("rand", 100, 1), ("rand", 257, 4), ("rand", 600, 2),
                                           ("clustered", 130, 2), ("degenerate", 97, 2), ("diag", 70, 1),
                                           ("graded", 120, 1), ("rand", 256, 3), ("rand", 452, 2),
-                                          ("rand", 1024, 1), ("clustered", 516, 2)])
+                                          ("rand", 1024, 1), ("clustered", 516, 2),
+                                          # orders padded to the vector width: diag(A, c I) inside
+                                          ("clustered", 301, 2), ("degenerate", 263, 2), ("graded", 259, 1),
+                                          ("diag", 258, 1), ("diag", 255, 1)])
 def test_syevd(cuda, prec, kind, n, batch):
     import torch
     from gammcor_amd import _lib
@@ -62,14 +65,15 @@ def test_syevd(cuda, prec, kind, n, batch):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_syevd_large_order_global_v(cuda, prec):
+@pytest.mark.parametrize("n", [10500, 10501])
+def test_syevd_large_order_global_v(cuda, prec, n):
     """n large enough that the panel kernel keeps the Householder vector in
     global memory instead of LDS (2 n sizeof(T) > 160 KiB for FP64 at
-    n = 10500; FP32 stays in LDS).  Checked on the GPU with torch's FP64
-    products (independent of this library)."""
+    n = 10500; FP32 stays in LDS); the odd order is solved padded to the
+    vector width.  Checked on the GPU with torch's FP64 products
+    (independent of this library)."""
     import torch
     from gammcor_amd import _lib
-    n = 10500
     dt = torch.float64 if prec == "f64" else torch.float32
     g = torch.Generator(device=cuda).manual_seed(5)
     a = torch.randn(n, n, dtype=torch.float64, device=cuda, generator=g)
