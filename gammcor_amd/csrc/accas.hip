@@ -851,19 +851,42 @@ __global__ void oa_delta_kernel(const double* partial, int np, double* delta, in
     delta[b] = 64.0 * 2.220446049250313e-16 * mx;
 }
 
-// E_ij = (s_ij + l_j r_ij) / (l_j - l_i), r = I - X^T X.  The cluster rule of
-// Ogita-Aishima (r_ij / 2 when |l_i - l_j| <= delta) is applied per element:
-// a pair counts as separated when its correction stays below 1/8 and its gap
-// is above the rounding level of the spectrum (delta[b] = 64 eps max|l|).
-// A close pair keeps only the orthogonalising r_ij / 2 here and is recorded
-// (lower index, upper index, Jacobi cosine and sine of its 2x2 block) so that
-// oa_pair_rot_kernel resolves it exactly after the update.
-// At most OA_MAXPAIRS close pairs per problem are recorded (the first `cap`,
-// cap <= OA_MAXPAIRS); npair keeps counting beyond, and a problem with more
-// pairs than recorded is treated as not converged (oa_policy.hpp).
-constexpr int OA_MAXPAIRS = 4096;
-constexpr int OA_EMAX_SLOTS = 64;   // per problem, spreads the atomics of the max
-constexpr int OA_ET = 32;           // tile of the E kernel
+// E_ij = (s_ij + l_j r_ij) / (l_j - l_i), r = I - X^T X (Ogita-Aishima).
+// The step contracts the error of a pair quadratically with a constant of
+// about max|l| / |l_i - l_j|, so a pair whose gap is small against the
+// spectrum and whose FP32 start is mixed by more than that constant allows
+// does not converge by these corrections; nor does a near-degenerate group of
+// three or more by 2x2 rotations computed from one S.  Such pairs are joined
+// into clusters (connected components), and each cluster is solved exactly:
+// the FP64 eigenvectors of its projected matrix H = S_CC + (l_i + l_j) r_ij / 2
+// (off the diagonal; l_i on it) rotate its columns after the update, which
+// keeps only the orthogonalising r_ij / 2 within a cluster (Ogita and Aishima,
+// "Iterative refinement for symmetric eigenvalue decomposition II: clustered
+// eigenvalues", Japan J. Indust. Appl. Math. 36 (2019)).
+//
+// A pair (i > j) joins when it is close (|l_i - l_j| not above 8 max(|num|)
+// or the rounding level delta = 64 eps max|l|), whatever its weights: the
+// modes the ERPA filter drops must be resolved among themselves too, or the
+// step of a kept mode against them (which takes their block of S as
+// diagonal) does not converge.  A separated pair joins when it is mixed by
+// |E| >= OA_JOIN and the mix matters: |E| rel >= OA_RHO, rel = |w_i - w_j| /
+// max w the weight difference (oa_weight) through which a rotation within
+// the pair moves the result.  (To first order the step leaves pair (i, j)
+// the error sum_k E_ki E_kj (l_j - l_k) / (l_j - l_i); with E_ki about
+// c / (l_k - l_i) from the FP32 start that is about |E_ij| times the largest
+// |E_ki| of the column, so once the strongly mixed pairs are solved as
+// clusters every other pair contracts by OA_JOIN or better per step.)
+// Clusters of up to OA_CMAX columns are solved by a workgroup Jacobi, larger
+// ones by the batched FP64 eigensolver.  At most `cap` pairs (<= OA_MAXPAIRS)
+// are recorded per problem; npair keeps counting beyond, and a problem with
+// more pairs than recorded, or with a cluster of more than half its columns,
+// is treated as not converged (oa_policy.hpp).
+constexpr int OA_MAXPAIRS = 262144;
+constexpr int OA_CMAX = 64;          // largest cluster the workgroup Jacobi solves; larger ones go to the eigensolver
+constexpr double OA_JOIN = 1.0 / 64;   // |E| from which a separated pair joins a cluster
+constexpr double OA_RHO = 1e-7;        // |E| rel from which such a pair matters to the result
+constexpr int OA_EMAX_SLOTS = 64;    // per problem, spreads the atomics of the max
+constexpr int OA_ET = 32;            // tile of the E kernel
 constexpr float OA_TOL = 1e-8f;      // a step whose corrections are all below this ends the refinement
 
 // The weight of eigenvalue l in what the solve makes of the eigenvectors:
@@ -878,59 +901,173 @@ __device__ inline double oa_weight(double l, int wmode)
     return om > 1e-3 && om < 1e8 ? 0.5 / om : 0.0;
 }
 
-// One element of E from s_ij and g_ij (the lower-triangle values of S and
-// G = X^T X); returns (|E_ij|, the predicted next correction of the
-// pair) for i != j ((0, 0) on the diagonal), +inf when not finite.  The
+// The pair terms of (i, j), i != j, from the lower-triangle values s_ij and
+// g_ij: the E_ij numerator and the gap, whether the pair is separated, its
+// weight difference rel and whether either mode has weight (used).  The
 // separated-or-close test is symmetric in (i, j) (the larger of
-// |s_ij + l_j r_ij| and |s_ij + l_i r_ij|), so E_ij and E_ji always take the
-// same branch.  The step converges quadratically with a constant of about
-// max|l| / gap, so a pair with a gap small against the spectrum keeps an
-// error far above E_ij^2 after the step; a rotation within the pair moves
-// the result in proportion to how differently the solve weighs the two
-// eigenvalues (oa_weight).  The prediction of a separated pair is therefore
-// E_ij^2 (max|l| / |l_j - l_i|) |w_i - w_j| / max(w_i, w_j) (0 for wmode 0
-// or two weights of 0): large for an ERPA mode the filter keeps next to one
-// it drops (a gap of 1e-5 of the spectrum in one system of the 64-system
-// workload), small for two kept modes of nearly equal omega.
-__device__ float2 oa_e_one(double sij, double gij, const double* lam, const double* delta, double* E, int n,
-                           unsigned* npair, int4* pairs, double2* rot, unsigned cap, int b, int i, int j, int wmode)
+// |s_ij + l_j r_ij| and |s_ij + l_i r_ij|).
+struct OaPair {
+    double num, den, nsym, rel;
+    bool sep, used;
+};
+__device__ inline OaPair oa_pair(double sij, double gij, const double* lamb, double delta, int i, int j, int wmode)
 {
+    OaPair q;
+    const double rij = -gij;
+    const double li = lamb[i], lj = lamb[j];
+    q.num = sij + lj * rij;
+    q.den = lj - li;
+    q.nsym = fmax(fabs(q.num), fabs(sij + li * rij));
+    const double wi = wmode ? oa_weight(li, wmode) : 1.0, wj = wmode ? oa_weight(lj, wmode) : 1.0;
+    const double wm = fmax(wi, wj);
+    q.used = wm > 0.0;
+    q.rel = wmode == 0 ? 1.0 : (q.used ? fabs(wi - wj) / wm : 0.0);
+    q.sep = fabs(q.den) > fmax(8.0 * q.nsym, delta);
+    return q;
+}
+
+// Pairs that join clusters, one thread per element of the strict lower
+// triangle (tiles of OA_ET x OA_ET, read along columns).
+__global__ __launch_bounds__(256) void oa_pairs_kernel(const double* S, const double* G, const double* lam,
+                                                       const double* delta, int n, unsigned* npair, int2* pairs,
+                                                       unsigned cap, int wmode)
+{
+    const int b = blockIdx.y, t = blockIdx.x;
+    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    while (I * (I + 1) / 2 > t) I--;
+    const int J = t - I * (I + 1) / 2;
     const size_t base = (size_t)b * n * n;
-    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const double dl = delta[b];
+    const double* lamb = lam + (size_t)b * n;
+    for (int idx = threadIdx.x; idx < OA_ET * OA_ET; idx += 256) {
+        const int i = OA_ET * I + idx % OA_ET, j = OA_ET * J + idx / OA_ET;
+        if (i >= n || j >= n || i <= j) continue;
+        const OaPair q = oa_pair(S[base + i + (size_t)j * n], G[base + i + (size_t)j * n], lamb, dl, i, j, wmode);
+        // (a NaN fails the separation test and joins)
+        const double es = q.nsym / fabs(q.den);
+        const bool join = !q.sep || (q.used && es >= OA_JOIN && es * q.rel >= OA_RHO);
+        if (join) {
+            const unsigned slot = atomicAdd(&npair[b], 1u);
+            if (slot < cap) pairs[(size_t)b * OA_MAXPAIRS + slot] = make_int2(j, i);
+        }
+    }
+}
+
+// Clusters of one problem from its recorded pairs (union-find in LDS, one
+// workgroup per problem; dynamic LDS 2 n ints).  Out: ccl[i] the cluster of
+// column i (-1 for none), the members of every cluster in ascending order
+// (cmem, from ctab[k].x, ctab[k].y of them) and, for one of at most OA_CMAX
+// columns, the offset ctab[k].z of its m x m rotation in qbuf; ncl = number
+// of clusters, cinfo = (largest cluster, 1 if one holds more than half the
+// columns, clusters above OA_CMAX, 0).
+__global__ __launch_bounds__(256) void oa_cluster_kernel(const unsigned* npair, const int2* pairs, unsigned cap, int n,
+                                                         int* ccl, int* cmem, int4* ctab, int* ncl, int* cinfo)
+{
+    extern __shared__ int csh[];
+    int* par = csh;
+    int* root = csh + n;
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        par[i] = i;
+        ccl[(size_t)b * n + i] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned np = min(npair[b], cap);
+        auto find = [&](int x) {
+            while (par[x] != x) {
+                par[x] = par[par[x]];   // path halving
+                x = par[x];
+            }
+            return x;
+        };
+        for (unsigned k = 0; k < np; k++) {
+            const int2 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
+            const int a = find(pq.x), c = find(pq.y);
+            if (a != c) par[max(a, c)] = min(a, c);   // the root is the smallest member
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+        int r = i;
+        while (par[r] != r) r = par[r];
+        root[i] = r;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) par[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) atomicAdd(&par[root[i]], 1);   // par: component sizes
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // ascending sweep: a root comes before its members, so it opens the
+        // cluster (par[root] = -(k + 1), root[root] = the fill count) and
+        // the members follow in ascending order
+        int nc = 0, tot = 0, qt = 0, big = 0, over = 0, nbig = 0;
+        for (int i = 0; i < n; i++) {
+            const int r = root[i];
+            int v = par[r];
+            if (r == i && v > 1) {
+                ctab[(size_t)b * n + nc] = make_int4(tot, v, qt, 0);
+                big = max(big, v);
+                over |= 2 * v > n;
+                nbig += v > OA_CMAX;
+                tot += v;
+                qt += v <= OA_CMAX ? v * v : 0;
+                par[r] = v = -(nc + 1);
+                root[r] = 0;
+                nc++;
+            }
+            if (v < 0) {
+                const int k = -v - 1;
+                cmem[(size_t)b * n + ctab[(size_t)b * n + k].x + root[r]++] = i;
+                ccl[(size_t)b * n + i] = k;
+            }
+        }
+        ncl[b] = nc;
+        cinfo[4 * b] = big;
+        cinfo[4 * b + 1] = over;
+        cinfo[4 * b + 2] = nbig;
+        cinfo[4 * b + 3] = 0;
+    }
+}
+
+// One element of E; returns (|E_ij|, the predicted next correction of the
+// pair) for i != j ((0, 0) on the diagonal and within a cluster), +inf when
+// not finite.  A separated pair takes the step's correction num / gap and
+// predicts |E_ij| min(1, |E_ij| max|l| / gap) rel (the error the step leaves,
+// quadratic where it contracts, weighted by how the solve uses the two
+// modes); a pair within a cluster, or a close pair
+// (one that could not be recorded, or of two modes of weight 0), keeps only
+// the orthogonalising r_ij / 2.  Pairs of two modes of weight 0 are not
+// counted.
+__device__ float2 oa_e_one(double sij, double gij, const double* lamb, double dl, double* E, const int* cclb, int n,
+                           size_t base, int i, int j, int wmode)
+{
     const double rij = (i == j ? 1.0 : 0.0) - gij;
-    const double li = lam[(size_t)b * n + i], lj = lam[(size_t)b * n + j];
-    const double num = sij + lj * rij, den = lj - li;
-    const double nsym = fmax(fabs(num), fabs(sij + li * rij));
+    if (i == j) {
+        E[base + i + (size_t)j * n] = 0.5 * rij;
+        return make_float2(0.f, 0.f);
+    }
+    const OaPair q = oa_pair(sij, gij, lamb, dl, i, j, wmode);
+    const int ci = cclb[i];
     double e, pred = 0.0;
-    if (i != j && fabs(den) > fmax(8.0 * nsym, delta[b])) {
-        e = num / den;
-        if (wmode) {
-            const double wi = oa_weight(li, wmode), wj = oa_weight(lj, wmode), wm = fmax(wi, wj);
-            if (wm > 0.0)
-                pred = e * e * (delta[b] * (1.0 / (64.0 * 2.220446049250313e-16))) / fabs(den) * (fabs(wi - wj) / wm);
+    bool count = q.used;
+    if (ci >= 0 && ci == cclb[j]) {
+        e = 0.5 * rij;
+        count = false;
+    } else if (q.sep) {
+        e = q.num / q.den;
+        if (wmode && q.used) {
+            const double kap = (dl * (1.0 / (64.0 * 2.220446049250313e-16))) / fabs(q.den);
+            pred = fabs(e) * fmin(1.0, fabs(e) * kap) * q.rel;
         }
     } else {
         e = 0.5 * rij;
-        if (i > j) {
-            // 2x2 block [[a, c], [c, b]] in the basis (x_lo, x_hi)
-            const double a = lam[(size_t)b * n + lo], bb = lam[(size_t)b * n + hi];
-            const double c = sij + 0.5 * (a + bb) * rij;
-            if (c != 0.0) {
-                const double tau = (bb - a) / (2.0 * c);
-                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double cs = 1.0 / sqrt(1.0 + t * t);
-                const unsigned slot = atomicAdd(&npair[b], 1u);
-                if (slot < cap) {
-                    pairs[(size_t)b * OA_MAXPAIRS + slot] = make_int4(lo, hi, 0, 0);
-                    rot[(size_t)b * OA_MAXPAIRS + slot] = make_double2(cs, t * cs);
-                }
-            }
-        }
     }
     E[base + i + (size_t)j * n] = e;
-    if (i == j) return make_float2(0.f, 0.f);
-    const float r = (float)fabs(e), q = (float)pred;
-    return make_float2(r <= 3.4028235e38f ? r : INFINITY, q <= 3.4028235e38f ? q : INFINITY);
+    const float r = (float)(count || !(fabs(e) <= 1e300) ? fabs(e) : 0.0), p = (float)pred;
+    return make_float2(r <= 3.4028235e38f ? r : INFINITY, p <= 3.4028235e38f ? p : INFINITY);
 }
 
 // E of one problem by OA_ET x OA_ET tile pairs (I >= J): S and G are valid
@@ -939,9 +1076,8 @@ __device__ float2 oa_e_one(double sij, double gij, const double* lam, const doub
 // and its mirror E(J, I) (E is not symmetric); E is written along columns
 // too.  One workgroup per tile pair and problem (blockIdx.y).
 __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double* G, const double* lam,
-                                                   const double* delta, double* E, int n, unsigned* npair, int4* pairs,
-                                                   double2* rot, unsigned cap, unsigned* emax, unsigned* pmax,
-                                                   int wmode)
+                                                   const double* delta, double* E, int n, const int* ccl,
+                                                   unsigned* emax, unsigned* pmax, int wmode)
 {
     __shared__ double st[OA_ET][OA_ET + 1], gt[OA_ET][OA_ET + 1];
     __shared__ float wmax[4], wpred[4];
@@ -952,6 +1088,9 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double
     const int J = t - I * (I + 1) / 2;
     const int r0 = OA_ET * I, c0 = OA_ET * J;
     const size_t base = (size_t)b * n * n;
+    const double* lamb = lam + (size_t)b * n;
+    const int* cclb = ccl + (size_t)b * n;
+    const double dl = delta[b];
     for (int idx = threadIdx.x; idx < OA_ET * OA_ET; idx += 256) {
         const int r = idx % OA_ET, c = idx / OA_ET;
         const int gi = r0 + r, gj = c0 + c;
@@ -970,13 +1109,11 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double
         const int r = idx % OA_ET, c = idx / OA_ET;
         // the lower tile: (i, j) = (r0 + r, c0 + c), i >= j
         int i = r0 + r, j = c0 + c;
-        if (i < n && j < n && i >= j)
-            take(oa_e_one(st[c][r], gt[c][r], lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode));
+        if (i < n && j < n && i >= j) take(oa_e_one(st[c][r], gt[c][r], lamb, dl, E, cclb, n, base, i, j, wmode));
         // its mirror: (i, j) = (c0 + r, r0 + c), i < j, from the lower element (j, i)
         i = c0 + r;
         j = r0 + c;
-        if (i < n && j < n && i < j)
-            take(oa_e_one(st[r][c], gt[r][c], lam, delta, E, n, npair, pairs, rot, cap, b, i, j, wmode));
+        if (i < n && j < n && i < j) take(oa_e_one(st[r][c], gt[r][c], lamb, dl, E, cclb, n, base, i, j, wmode));
     }
     for (int o = 32; o > 0; o >>= 1) {
         my.x = fmaxf(my.x, __shfl_xor(my.x, o, 64));
@@ -994,51 +1131,236 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double
     }
 }
 
-// x_lo <- cs x_lo - sn x_hi, x_hi <- sn x_lo + cs x_hi for every recorded
-// close pair (in record order), one workgroup per problem.
-__global__ __launch_bounds__(256) void oa_pair_rot_kernel(double* X, int n, const unsigned* npair, const int4* pairs,
-                                                          const double2* rot, unsigned cap)
+// Each cluster's projected matrix H (m x m, m <= OA_CMAX) diagonalised by
+// cyclic Jacobi in FP64 (round-robin order: m/2 disjoint rotations at once,
+// H <- J^T H J, Q <- Q J), rotations below a hundredth of the spectrum's
+// rounding level skipped; its eigenvalues, ascending, become the members'
+// eigenvalue estimates (in member order) and the matching columns of Q are
+// stored at qbuf + ctab.z (column-major m x m).  Workgroups stride over the
+// clusters of problem blockIdx.y.
+__global__ __launch_bounds__(256) void oa_cluster_solve_kernel(const double* S, const double* G, double* lam,
+                                                               const double* delta, int n, const int* ncl,
+                                                               const int4* ctab, const int* cmem, double* qbuf)
 {
-    const int b = blockIdx.x;
-    const unsigned np = min(npair[b], cap);
-    double* x = X + (size_t)b * n * n;
-    for (unsigned k = 0; k < np; k++) {
-        const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
-        const double2 cr = rot[(size_t)b * OA_MAXPAIRS + k];
-        for (int r = threadIdx.x; r < n; r += 256) {
-            const double u = x[r + (size_t)pq.x * n], v = x[r + (size_t)pq.y * n];
-            x[r + (size_t)pq.x * n] = cr.x * u - cr.y * v;
-            x[r + (size_t)pq.y * n] = cr.y * u + cr.x * v;
+    __shared__ double H[OA_CMAX][OA_CMAX + 1], Q[OA_CMAX][OA_CMAX + 1];
+    __shared__ double rc[OA_CMAX / 2], rs[OA_CMAX / 2];
+    __shared__ int rp[OA_CMAX / 2], rq[OA_CMAX / 2], idx[OA_CMAX], rank[OA_CMAX];
+    __shared__ int any;
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const size_t base = (size_t)b * n * n;
+    double* lamb = lam + (size_t)b * n;
+    const double tol = 0.01 * delta[b] / 64.0;   // eps max|l| / 100
+    for (int k = blockIdx.x; k < ncl[b]; k += gridDim.x) {
+        const int4 ct = ctab[(size_t)b * n + k];
+        const int m = ct.y;
+        if (m > OA_CMAX) continue;
+        __syncthreads();
+        for (int u = tid; u < m; u += 256) idx[u] = cmem[(size_t)b * n + ct.x + u];
+        __syncthreads();
+        for (int e = tid; e < m * m; e += 256) {
+            const int r = e % m, c = e / m;
+            const int i = idx[r], j = idx[c];
+            double h;
+            if (r == c) {
+                h = lamb[i];
+            } else {
+                const int hi = max(i, j), lo = min(i, j);
+                h = S[base + hi + (size_t)lo * n] - 0.5 * (lamb[i] + lamb[j]) * G[base + hi + (size_t)lo * n];
+            }
+            H[r][c] = h;
+            Q[r][c] = r == c ? 1.0 : 0.0;
+        }
+        const int mp = m + (m & 1);   // even: index m (if any) is a dummy that never rotates
+        for (int sweep = 0; sweep < 30; sweep++) {
+            __syncthreads();   // every thread has read `any` of the last sweep
+            if (tid == 0) any = 0;
+            __syncthreads();
+            for (int rd = 0; rd < mp - 1; rd++) {
+                // round rd: positions 1..mp-1 rotated by rd, position 0 fixed
+                if (tid < mp / 2) {
+                    const int a = tid == 0 ? 0 : 1 + (tid - 1 + rd) % (mp - 1);
+                    const int c = 1 + (mp - 2 - tid + rd) % (mp - 1);
+                    const int p = min(a, c), q = max(a, c);
+                    double cs = 1.0, sn = 0.0;
+                    if (q < m) {
+                        const double hpq = H[p][q];
+                        if (fabs(hpq) > tol) {
+                            const double tau = (H[q][q] - H[p][p]) / (2.0 * hpq);
+                            const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                            cs = 1.0 / sqrt(1.0 + tt * tt);
+                            sn = tt * cs;
+                            any = 1;
+                        }
+                    }
+                    rp[tid] = p;
+                    rq[tid] = q < m ? q : p;
+                    rc[tid] = cs;
+                    rs[tid] = sn;
+                }
+                __syncthreads();
+                // rows p, q of H (the pairs are disjoint)
+                for (int e = tid; e < (mp / 2) * m; e += 256) {
+                    const int t2 = e / m, j = e % m;
+                    const double sn = rs[t2];
+                    if (sn == 0.0) continue;
+                    const int p = rp[t2], q = rq[t2];
+                    const double cs = rc[t2], hp = H[p][j], hq = H[q][j];
+                    H[p][j] = cs * hp - sn * hq;
+                    H[q][j] = sn * hp + cs * hq;
+                }
+                __syncthreads();
+                // columns p, q of H and Q
+                for (int e = tid; e < (mp / 2) * m; e += 256) {
+                    const int t2 = e / m, j = e % m;
+                    const double sn = rs[t2];
+                    if (sn == 0.0) continue;
+                    const int p = rp[t2], q = rq[t2];
+                    const double cs = rc[t2];
+                    const double hp = H[j][p], hq = H[j][q];
+                    H[j][p] = cs * hp - sn * hq;
+                    H[j][q] = sn * hp + cs * hq;
+                    const double qp = Q[j][p], qq = Q[j][q];
+                    Q[j][p] = cs * qp - sn * qq;
+                    Q[j][q] = sn * qp + cs * qq;
+                }
+                __syncthreads();
+            }
+            const int go = any;
+            if (!go) break;
+        }
+        // ascending order of the eigenvalues (ties by position)
+        for (int u = tid; u < m; u += 256) {
+            const double v = H[u][u];
+            int rk = 0;
+            for (int w = 0; w < m; w++) rk += H[w][w] < v || (H[w][w] == v && w < u);
+            rank[u] = rk;
         }
         __syncthreads();
+        for (int u = tid; u < m; u += 256) lamb[idx[rank[u]]] = H[u][u];
+        double* qk = qbuf + (size_t)b * OA_CMAX * n + ct.z;
+        for (int e = tid; e < m * m; e += 256) {
+            const int r = e % m, c = e / m;
+            qk[r + (size_t)rank[c] * m] = Q[r][c];
+        }
     }
 }
 
-// Eigenvalue estimates of the rotated pairs: a - t c, b + t c.
-__global__ void oa_pair_lam_kernel(double* lam, const double* S, const double* G, int n, const unsigned* npair,
-                                   const int4* pairs, const double2* rot, unsigned cap)
+// X_C <- X_C Q_C for every cluster of problem blockIdx.y, on the 64 rows
+// from 64 blockIdx.x (the cluster's columns staged in LDS, then written).
+__global__ __launch_bounds__(256) void oa_cluster_apply_kernel(double* X, int n, const int* ncl, const int4* ctab,
+                                                               const int* cmem, const double* qbuf)
 {
-    const int b = blockIdx.x;
-    if (threadIdx.x != 0) return;
-    const unsigned np = min(npair[b], cap);
-    for (unsigned k = 0; k < np; k++) {
-        const int4 pq = pairs[(size_t)b * OA_MAXPAIRS + k];
-        const double2 cr = rot[(size_t)b * OA_MAXPAIRS + k];
-        const size_t at = (size_t)b * n * n + pq.y + (size_t)pq.x * n;
-        double* a = &lam[(size_t)b * n + pq.x];
-        double* bb = &lam[(size_t)b * n + pq.y];
-        const double c = S[at] + 0.5 * (*a + *bb) * (-G[at]);
-        const double t = cr.y / cr.x;
-        *a -= t * c;
-        *bb += t * c;
+    __shared__ double xs[64][OA_CMAX + 1];
+    __shared__ double qs[OA_CMAX * OA_CMAX];
+    __shared__ int idx[OA_CMAX];
+    const int b = blockIdx.y, r0 = 64 * blockIdx.x, tid = threadIdx.x;
+    const int nr = min(64, n - r0);
+    double* x = X + (size_t)b * n * n;
+    for (int k = 0; k < ncl[b]; k++) {
+        const int4 ct = ctab[(size_t)b * n + k];
+        const int m = ct.y;
+        if (m > OA_CMAX) continue;
+        __syncthreads();
+        for (int u = tid; u < m; u += 256) idx[u] = cmem[(size_t)b * n + ct.x + u];
+        const double* qk = qbuf + (size_t)b * OA_CMAX * n + ct.z;
+        for (int e = tid; e < m * m; e += 256) qs[e] = qk[e];
+        __syncthreads();
+        for (int e = tid; e < 64 * m; e += 256) {
+            const int r = e % 64, c = e / 64;
+            xs[r][c] = r < nr ? x[r0 + r + (size_t)idx[c] * n] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * m; e += 256) {
+            const int r = e % 64, c = e / 64;
+            if (r >= nr) continue;
+            double acc = 0.0;
+            for (int u = 0; u < m; u++) acc += xs[r][u] * qs[u + c * m];
+            x[r0 + r + (size_t)idx[c] * n] = acc;
+        }
     }
 }
 
-// One Ogita-Aishima step: X <- X + X E.  Buffers: AX, S, G, Xn (B x n x n).
+// Clusters above OA_CMAX columns, solved by the batched FP64 eigensolver in
+// chunks of at most OA_BIGCHUNK of one padded order mb.  bigl[s] = (problem,
+// cluster, 0, 0) of chunk slot s.
+constexpr int OA_BIGCHUNK = 256;
+
+// H of chunk slot blockIdx.x (mb x mb at H + slot mb^2): the cluster's
+// projected matrix, padded with the diagonal value c above its Gershgorin
+// bound so the pad eigenvalues come last and the first m eigenvectors have
+// zero pad rows.
+__global__ __launch_bounds__(256) void oa_big_h_kernel(const double* S, const double* G, const double* lam, int n,
+                                                       const int4* bigl, const int4* ctab, const int* cmem, double* H,
+                                                       int mb)
+{
+    __shared__ double red[4];
+    const int4 L = bigl[blockIdx.x];
+    const int b = L.x;
+    const int4 ct = ctab[(size_t)b * n + L.y];
+    const int m = ct.y;
+    const int* idx = cmem + (size_t)b * n + ct.x;
+    const size_t base = (size_t)b * n * n;
+    const double* lamb = lam + (size_t)b * n;
+    double* h = H + (size_t)blockIdx.x * mb * mb;
+    auto hval = [&](int r, int c) -> double {
+        const int i = idx[r], j = idx[c];
+        if (r == c) return lamb[i];
+        const int hi = max(i, j), lo = min(i, j);
+        return S[base + hi + (size_t)lo * n] - 0.5 * (lamb[i] + lamb[j]) * G[base + hi + (size_t)lo * n];
+    };
+    double gm = 0.0;
+    for (int c = threadIdx.x; c < m; c += 256) {
+        double cs = 0.0;
+        for (int r = 0; r < m; r++) cs += fabs(hval(r, c));
+        gm = fmax(gm, cs);
+    }
+    for (int o = 32; o > 0; o >>= 1) gm = fmax(gm, __shfl_xor(gm, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = gm;
+    __syncthreads();
+    gm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const double pad = gm * (1.0 + 0.0009765625) + 2.2250738585072014e-308;
+    for (int64_t e = threadIdx.x; e < (int64_t)mb * mb; e += 256) {
+        const int r = (int)(e % mb), c = (int)(e / mb);
+        h[e] = (r < m && c < m) ? hval(r, c) : (r == c ? pad : 0.0);
+    }
+}
+
+// The eigenvalues of chunk slot blockIdx.x (ascending, pads last) become the
+// members' estimates.
+__global__ void oa_big_lam_kernel(const double* w, int mb, const int4* bigl, const int4* ctab, const int* cmem,
+                                  double* lam, int n)
+{
+    const int4 L = bigl[blockIdx.x];
+    const int b = L.x;
+    const int4 ct = ctab[(size_t)b * n + L.y];
+    for (int k = threadIdx.x; k < ct.y; k += blockDim.x)
+        lam[(size_t)b * n + cmem[(size_t)b * n + ct.x + k]] = w[(size_t)blockIdx.x * mb + k];
+}
+
+// XC (n x m) = the cluster's columns of X (to = 0), or back (to = 1).
+__global__ void oa_big_move_kernel(double* X, int n, const int* idx, int m, double* XC, int to)
+{
+    const int64_t total = (int64_t)n * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        double* xp = X + r + (size_t)idx[c] * n;
+        if (to) *xp = XC[e];
+        else XC[e] = *xp;
+    }
+}
+
+// One Ogita-Aishima step: X <- X + X E, then the cluster rotations.
+// Buffers: AX, S, G, Xn (B x n x n).
 struct OaPairs {
     unsigned* npair;    // B
-    int4* pairs;        // B x OA_MAXPAIRS
-    double2* rot;       // B x OA_MAXPAIRS
+    int2* pairs;        // B x OA_MAXPAIRS: (j, i), i > j
+    int* ccl;           // B x n: cluster of each column, -1 for none
+    int* cmem;          // B x n: members of the clusters
+    int4* ctab;         // B x n: (first member, size, rotation offset) per cluster
+    int* ncl;           // B
+    int* cinfo;         // B x 4: largest cluster, one above half the columns, clusters above OA_CMAX
+    int4* bigl;         // B x OA_BIGMAX: the clusters above OA_CMAX of one eigensolver chunk
+    double* qbuf;       // B x OA_CMAX x n: the clusters' rotations
     unsigned* emax;     // B x OA_EMAX_SLOTS: float bits of max |E_ij| (i != j) of the last step
     unsigned* pmax;     // B x OA_EMAX_SLOTS: float bits of the largest predicted next correction
     unsigned cap = OA_MAXPAIRS;   // pairs recorded per problem (<= OA_MAXPAIRS)
@@ -1088,7 +1410,8 @@ __global__ void f64_to_f16x2_kernel(const double* a, __half* hi, __half* lo, int
 static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, double*& X, double*& Xn, double* lam,
                      double* AX,
                      double* S, double* G, double* partial, int npart, double* delta, const OaPairs& op, float* X32,
-                     float* E32, float* P32, float half_from, double* dscale, std::vector<float>& pe)
+                     float* E32, float* P32, float half_from, double* dscale, std::vector<float>& pe,
+                     const EigSolveAt* cluster_eig)
 {
     const size_t nn = (size_t)n * n;
     GemmShape gnn{n, n, n, false, false, n, n, n, (int64_t)nn, (int64_t)nn, (int64_t)nn, B, 0};
@@ -1102,18 +1425,85 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
     GMC_HIP(hipMemsetAsync(op.emax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     GMC_HIP(hipMemsetAsync(op.pmax, 0, (size_t)B * OA_EMAX_SLOTS * sizeof(unsigned), st));
     const int nte = (n + OA_ET - 1) / OA_ET;
-    hipLaunchKernelGGL(oa_e_kernel, dim3(nte * (nte + 1) / 2, B), dim3(256), 0, st, S, G, lam, delta, AX, n,
-                       op.npair, op.pairs, op.rot, op.cap, op.emax, op.pmax, wmode);
+    const dim3 gtri(nte * (nte + 1) / 2, B);
+    // the pairs that join clusters, the clusters, then E with them
+    hipLaunchKernelGGL(oa_pairs_kernel, gtri, dim3(256), 0, st, S, G, lam, delta, n, op.npair, op.pairs, op.cap,
+                       wmode);
+    const size_t clds = 2 * (size_t)n * sizeof(int);
+    GMC_CHECK(clds <= 160 * 1024, "ERPA mixed: order %d too large for the cluster kernel's LDS", n);
+    static bool clds_set = false;
+    if (!clds_set) {
+        GMC_HIP(hipFuncSetAttribute((const void*)oa_cluster_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+        clds_set = true;
+    }
+    hipLaunchKernelGGL(oa_cluster_kernel, dim3(B), dim3(256), clds, st, op.npair, op.pairs, op.cap, n, op.ccl,
+                       op.cmem, op.ctab, op.ncl, op.cinfo);
+    hipLaunchKernelGGL(oa_e_kernel, gtri, dim3(256), 0, st, S, G, lam, delta, AX, n, op.ccl, op.emax, op.pmax, wmode);
     GMC_HIP(hipGetLastError());
     // each problem's largest off-diagonal correction and prediction
     // (non-negative floats order as their bits; +inf marks a non-finite
-    // element) and pair count
+    // element), pair count and clusters
     std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS), pm((size_t)B * OA_EMAX_SLOTS), np(B);
+    std::vector<int> ci(4 * (size_t)B), ncl(B);
     GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipMemcpyAsync(pm.data(), op.pmax, pm.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipMemcpyAsync(np.data(), op.npair, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipMemcpyAsync(ci.data(), op.cinfo, ci.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    GMC_HIP(hipMemcpyAsync(ncl.data(), op.ncl, B * sizeof(int), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipStreamSynchronize(st));
     pe.assign(B, 0.f);
+    // Clusters above OA_CMAX columns: (problem, cluster, size, first member),
+    // largest first, in eigensolver chunks of one padded order that fit the
+    // old X's space (free once the update has swapped X and Xn).  A problem
+    // whose cluster cannot be solved so stays unconverged.
+    std::vector<char> bad(B, 0);
+    std::vector<int4> bigs;
+    for (int b = 0; b < B; b++) {
+        unsigned mxb = 0, pxb = 0;
+        for (int k = 0; k < OA_EMAX_SLOTS; k++) {
+            mxb = std::max(mxb, em[(size_t)b * OA_EMAX_SLOTS + k]);
+            pxb = std::max(pxb, pm[(size_t)b * OA_EMAX_SLOTS + k]);
+        }
+        // (+inf bits or above: a non-finite step, whose clusters are not solved)
+        if (ci[4 * b + 1] || mxb >= 0x7f800000u || pxb >= 0x7f800000u) bad[b] = 1;
+        if (bad[b] || ci[4 * b + 2] == 0) continue;
+        if (!cluster_eig) {
+            bad[b] = 1;
+            continue;
+        }
+        std::vector<int4> tab(ncl[b]);
+        GMC_HIP(hipMemcpy(tab.data(), op.ctab + (size_t)b * n, tab.size() * sizeof(int4), hipMemcpyDeviceToHost));
+        for (int k = 0; k < ncl[b]; k++)
+            if (tab[k].y > OA_CMAX) bigs.push_back(make_int4(b, k, tab[k].y, tab[k].x));
+    }
+    std::stable_sort(bigs.begin(), bigs.end(), [](const int4& u, const int4& v) { return u.z > v.z; });
+    struct BigChunk {
+        int mb;
+        size_t i0, c;
+    };
+    std::vector<BigChunk> chunks;
+    {
+        const size_t avail = (size_t)B * nn * sizeof(double);
+        auto need = [&](int mb, int c) -> size_t {
+            const size_t ws = eig_workspace_bytes<double>(mb, c);
+            if (ws == SIZE_MAX) return SIZE_MAX;
+            return align_up((size_t)c * mb * mb * sizeof(double)) + align_up((size_t)c * mb * sizeof(double)) + ws;
+        };
+        size_t i0 = 0;
+        while (i0 < bigs.size()) {
+            const int mb = (bigs[i0].z + 31) / 32 * 32;
+            int c = (int)std::min<size_t>(OA_BIGCHUNK, bigs.size() - i0);
+            while (c > 1 && need(mb, c) > avail) c = (c + 1) / 2;
+            if (need(mb, c) > avail) {
+                bad[bigs[i0].x] = 1;
+                i0++;
+                continue;
+            }
+            chunks.push_back({mb, i0, (size_t)c});
+            i0 += c;
+        }
+    }
     // Each problem's update is scaled by its own largest correction: one
     // batch holds problems at every stage of convergence (the batching form
     // mixes systems), and a batch-wide scale would push a converging
@@ -1130,7 +1520,7 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
         float e, q;
         std::memcpy(&e, &mxb, 4);
         std::memcpy(&q, &pxb, 4);
-        pe[b] = oa_effective(oa_measure(e, q), np[b], op.cap);
+        pe[b] = bad[b] ? INFINITY : oa_effective(oa_measure(e, q), np[b], op.cap);
         eb[b] = e <= 3.4028235e38f ? e : 1.0f;
         if (e <= 3.4028235e38f) emin = std::min(emin, e);
     }
@@ -1174,12 +1564,49 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
         GMC_TRY(gemm_f16(st, gnn, lo, Xh, El, 1.0f, P32));
         GMC_TRY(gemm_f16(st, gnn, lo, Xl, Eh, 1.0f, P32));
     }
+    // the clusters' rotations (from S, G and the step's eigenvalues) and
+    // eigenvalue estimates, applied after the update
+    hipLaunchKernelGGL(oa_cluster_solve_kernel, dim3(64, B), dim3(256), 0, st, S, G, lam, delta, n, op.ncl, op.ctab,
+                       op.cmem, op.qbuf);
     hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, cnt, snn, s_b);
     GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
-    hipLaunchKernelGGL(oa_pair_rot_kernel, dim3(B), dim3(256), 0, st, X, n, op.npair, op.pairs, op.rot, op.cap);
-    hipLaunchKernelGGL(oa_pair_lam_kernel, dim3(B), dim3(64), 0, st, lam, S, G, n, op.npair, op.pairs, op.rot, op.cap);
+    hipLaunchKernelGGL(oa_cluster_apply_kernel, dim3((n + 63) / 64, B), dim3(256), 0, st, X, n, op.ncl, op.ctab,
+                       op.cmem, op.qbuf);
     GMC_HIP(hipGetLastError());
+    // the large clusters: H in the old X's space, its eigenvectors rotate
+    // the cluster's columns (moved through AX, free after the update)
+    for (const BigChunk& ch : chunks) {
+        if (bad[bigs[ch.i0].x]) continue;
+        std::vector<int4> list(ch.c);
+        for (size_t u = 0; u < ch.c; u++) list[u] = make_int4(bigs[ch.i0 + u].x, bigs[ch.i0 + u].y, 0, 0);
+        GMC_HIP(hipMemcpyAsync(op.bigl, list.data(), ch.c * sizeof(int4), hipMemcpyHostToDevice, st));
+        const int mb = ch.mb, c = (int)ch.c;
+        double* H = Xn;
+        double* w = H + align_up((size_t)c * mb * mb * sizeof(double)) / sizeof(double);
+        void* ws = w + align_up((size_t)c * mb * sizeof(double)) / sizeof(double);
+        const size_t wsb = eig_workspace_bytes<double>(mb, c);
+        hipLaunchKernelGGL(oa_big_h_kernel, dim3(c), dim3(256), 0, st, S, G, lam, n, op.bigl, op.ctab, op.cmem, H, mb);
+        GMC_HIP(hipGetLastError());
+        GMC_TRY((*cluster_eig)(H, w, mb, c, ws, wsb));
+        hipLaunchKernelGGL(oa_big_lam_kernel, dim3(c), dim3(256), 0, st, w, mb, op.bigl, op.ctab, op.cmem, lam, n);
+        for (size_t u = 0; u < ch.c; u++) {
+            const int4 bg = bigs[ch.i0 + u];
+            const int m = bg.z;
+            const int* idx = op.cmem + (size_t)bg.x * n + bg.w;
+            double* Xb = X + (size_t)bg.x * nn;
+            double* XC = AX;
+            double* XO = AX + (size_t)n * m;
+            const unsigned gm = (unsigned)std::min<int64_t>(65536, ((int64_t)n * m + 255) / 256);
+            hipLaunchKernelGGL(oa_big_move_kernel, dim3(gm), dim3(256), 0, st, Xb, n, idx, m, XC, 0);
+            GemmShape g{n, m, m, false, false, n, mb, n, 0, 0, 0, 1, 0};
+            GMC_TRY(gemm_big<double>(st, g, 1.0, XC, H + (size_t)u * mb * mb, 0.0, XO));
+            hipLaunchKernelGGL(oa_big_move_kernel, dim3(gm), dim3(256), 0, st, Xb, n, idx, m, XO, 1);
+            GMC_HIP(hipGetLastError());
+        }
+        // `list` must outlive its copy
+        GMC_HIP(hipStreamSynchronize(st));
+    }
     return 0;
 }
 
@@ -1197,7 +1624,8 @@ size_t erpa_mixed_bytes(int n, int batch)
     const size_t nn = (size_t)n * n, B = batch;
     return carve_bytes({B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8, B * nn * 8,
                         B * nn * 4, B * n * 4, B * n * 8, B * n * 8, B * 8, B * 1024 * 3 * 8, B * 8, B * 4,
-                        B * (size_t)OA_MAXPAIRS * 16, B * (size_t)OA_MAXPAIRS * 16, B * nn * 4, B * nn * 4,
+                        B * (size_t)OA_MAXPAIRS * 8, B * n * 4, B * n * 4, B * n * 16, B * 4, 4 * B * 4,
+                        B * (size_t)OA_CMAX * n * 8, (size_t)OA_BIGCHUNK * 16, B * nn * 4, B * nn * 4,
                         B * (size_t)OA_EMAX_SLOTS * 4, B * (size_t)OA_EMAX_SLOTS * 4, B * n * 8,
                         potrf_work_bytes(n, batch), B * 4, 2 * B * 8});
 }
@@ -1335,7 +1763,7 @@ __global__ void pad_diag_set_kernel(const BatchSys* sys, int nalpha, double* A, 
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
                        int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from,
-                       const ErpaBatchTables* bt)
+                       const ErpaBatchTables* bt, const EigSolveAt* cluster_eig)
 {
     // B problems: the alpha points, or (batching form) every system at every alpha point
     const int B = bt ? bt->nsys * nalpha : nalpha;
@@ -1359,8 +1787,14 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     double* delta = ar.take<double>(B);
     OaPairs op;
     op.npair = ar.take<unsigned>(B);
-    op.pairs = ar.take<int4>((size_t)B * OA_MAXPAIRS);
-    op.rot = ar.take<double2>((size_t)B * OA_MAXPAIRS);
+    op.pairs = ar.take<int2>((size_t)B * OA_MAXPAIRS);
+    op.ccl = ar.take<int>((size_t)B * n);
+    op.cmem = ar.take<int>((size_t)B * n);
+    op.ctab = ar.take<int4>((size_t)B * n);
+    op.ncl = ar.take<int>(B);
+    op.cinfo = ar.take<int>(4 * (size_t)B);
+    op.qbuf = ar.take<double>((size_t)B * OA_CMAX * n);
+    op.bigl = ar.take<int4>(OA_BIGCHUNK);
     float* X32 = ar.take<float>(B * nn);
     float* P32 = ar.take<float>(B * nn);
     op.emax = ar.take<unsigned>((size_t)B * OA_EMAX_SLOTS);
@@ -1390,7 +1824,17 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
             std::memcpy(&f, &u, 4);
             mx = std::max(mx, f);
         }
-        fprintf(stderr, "[gmc] OA %s step %d: %u close pairs over %d problems, max |E| %.3e\n", what, it, tot, B, mx);
+        std::vector<int> nc(B), ci(4 * (size_t)B);
+        GMC_HIP(hipMemcpyAsync(nc.data(), op.ncl, B * sizeof(int), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipMemcpyAsync(ci.data(), op.cinfo, ci.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        GMC_HIP(hipStreamSynchronize(st));
+        int ctot = 0, big = 0;
+        for (int b = 0; b < B; b++) {
+            ctot += nc[b];
+            big = std::max(big, ci[4 * b]);
+        }
+        fprintf(stderr, "[gmc] OA %s step %d: %u pairs in %d clusters (largest %d) over %d problems, max |E| %.3e\n",
+                what, it, tot, ctot, big, B, mx);
         return 0;
     };
     // Largest correction of the last step, per problem (oa_effective)
@@ -1405,7 +1849,7 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         left.clear();
         for (int it = 0; it < n_max; it++) {
             GMC_TRY(oa_refine(st, n, B, wmode, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32,
-                              P32, half_from, dscale, pe));
+                              P32, half_from, dscale, pe, cluster_eig));
             GMC_TRY(report(what, it));
             if (stats && atoi(getenv("GMC_OA_STATS")) >= 2) {
                 // every problem's correction of the step
@@ -1417,9 +1861,10 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
             if (d.next == OaNext::Continue) continue;
             if (d.next == OaNext::Resolve) left = d.left;
             GMC_CHECK(d.next != OaNext::Fail,
-                      "ERPA mixed: refinement of %s left %d problem(s) (first %d) with non-finite corrections or more "
-                      "than %u close pairs after %d steps; GMC_MIXED solves such problems again in FP64", what,
-                      (int)d.left.size(), d.left.empty() ? -1 : d.left[0], op.cap, n_max);
+                      "ERPA mixed: refinement of %s left %d problem(s) (first %d) with non-finite corrections, more "
+                      "than %u close pairs or a cluster of more than %d columns after %d steps; GMC_MIXED solves such "
+                      "problems again in FP64", what, (int)d.left.size(), d.left.empty() ? -1 : d.left[0], op.cap,
+                      OA_CMAX, n_max);
             break;
         }
         return 0;

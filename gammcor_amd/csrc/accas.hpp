@@ -121,6 +121,10 @@ constexpr float OAC_TOL = 1e-11f;
 // FP64 eigensolve of nb problems side by side (stride n*n, overwritten by
 // their eigenvectors; w gets the eigenvalues, ascending).
 using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
+// FP64 eigensolve of nb problems of order m side by side (stride m*m,
+// overwritten by their eigenvectors; w gets the eigenvalues, ascending) in
+// the caller's workspace ws of ws_bytes bytes (eig_workspace_bytes).
+using EigSolveAt = std::function<int(double* A, double* w, int m, int nb, void* ws, size_t ws_bytes)>;
 
 // Mixed path: FP32 eigen solves refined by n_min to n_max Ogita-Aishima
 // steps in FP64 (more than n_min while the corrections stay at or above
@@ -129,11 +133,12 @@ using EigSolve64 = std::function<int(double* A, double* w, int nb)>;
 // n_max steps, or earlier once every such problem is at or above OA_GIVEUP)
 // are solved again by `resolve` from their FP64 matrices.  Steps with a
 // largest correction of at least half_from form the update in half
-// precision (OA_HALF).
+// precision (OA_HALF).  `cluster_eig` solves the refinement's clusters of
+// more than 64 columns (without it a problem with one is not converged).
 size_t erpa_mixed_bytes(int n, int batch);
 int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& ab, const double* alphas, int nalpha,
                        const double* Gw, const double* wdiag, void* scratch, size_t scratch_bytes, ErpaOut& out,
                        int n_min, int n_max, float oa_extend, const EigSolve64* resolve, float half_from,
-                       const ErpaBatchTables* bt = nullptr);
+                       const ErpaBatchTables* bt = nullptr, const EigSolveAt* cluster_eig = nullptr);
 
 }  // namespace gmc

@@ -212,16 +212,13 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
             const size_t mb = erpa_mixed_bytes(npad, B);
             GMC_TRY(st.erpaws.reserve(align_up(mb)));
             const EigSolve64 resolve = [&](double* A, double* w, int nb) -> int {
-                EigPlan<double>* p64 = nullptr;
-                GMC_TRY(st.eig_plan_for<double>(npad, nb, &p64));
-                GMC_TRY(st.resolve.reserve(p64->ws_bytes));
-                GMC_TRY(eig_bind<double>(p64, st.resolve.ptr));
-                GMC_TRY(syevd_batched<double>(s, *p64, A, w));
-                GMC_TRY(eig_check_error<double>(s, *p64));
-                return 0;
+                return st.resolve64(s, npad, A, w, nb);
+            };
+            const EigSolveAt cluster_eig = [&](double* A, double* w, int m, int nb, void* ws, size_t wsb) -> int {
+                return st.solve64_at(s, m, A, w, nb, ws, wsb);
             };
             GMC_TRY(erpa_batched_mixed(s, *ep, npad, none, d_al, nalpha, nullptr, nullptr, st.erpaws.ptr, mb, outs[g],
-                                       3, OA_MAX_STEPS, OA_EXTEND, &resolve, OA_HALF, &bt));
+                                       3, OA_MAX_STEPS, OA_EXTEND, &resolve, OA_HALF, &bt, &cluster_eig));
             GMC_TRY(eig_check_error<float>(s, *ep));
         }
         return 0;

@@ -69,6 +69,43 @@ int State::eig_plan_for(int n, int batch, EigPlan<T>** out)
 template int State::eig_plan_for<double>(int, int, EigPlan<double>**);
 template int State::eig_plan_for<float>(int, int, EigPlan<float>**);
 
+int State::solve64_at(hipStream_t s, int n, double* A, double* w, int nb, void* ws, size_t ws_bytes)
+{
+    EigPlan<double>* p = nullptr;
+    GMC_TRY(eig_plan_for<double>(n, nb, &p));
+    GMC_CHECK(p->ws_bytes <= ws_bytes, "eigensolve: %zu bytes of workspace given, %zu needed", ws_bytes, p->ws_bytes);
+    GMC_TRY(eig_bind<double>(p, ws));
+    GMC_TRY(syevd_batched<double>(s, *p, A, w));
+    GMC_TRY(eig_check_error<double>(s, *p));
+    return 0;
+}
+
+int State::resolve64(hipStream_t s, int n, double* A, double* w, int nb)
+{
+    GMC_CHECK(nb >= 1, "resolve: %d problems", nb);
+    // halve the chunk until its workspace fits what `resolve` holds plus
+    // nine tenths of the free memory
+    int chunk = nb;
+    for (;;) {
+        EigPlan<double>* p64 = nullptr;
+        GMC_TRY(eig_plan_for<double>(n, chunk, &p64));
+        size_t fr = 0, tot = 0;
+        GMC_HIP(hipMemGetInfo(&fr, &tot));
+        if (chunk == 1 || p64->ws_bytes <= resolve.bytes + fr / 10 * 9) break;
+        chunk = (chunk + 1) / 2;
+    }
+    for (int c0 = 0; c0 < nb; c0 += chunk) {
+        const int c = std::min(chunk, nb - c0);
+        EigPlan<double>* p64 = nullptr;
+        GMC_TRY(eig_plan_for<double>(n, c, &p64));
+        GMC_TRY(resolve.reserve(p64->ws_bytes));
+        GMC_TRY(eig_bind<double>(p64, resolve.ptr));
+        GMC_TRY(syevd_batched<double>(s, *p64, A + (size_t)c0 * n * n, w + (size_t)c0 * n));
+        GMC_TRY(eig_check_error<double>(s, *p64));
+    }
+    return 0;
+}
+
 // The stream of a device-pointer entry point: the caller's, and for NULL the
 // null stream itself, so the call is ordered after the caller's earlier
 // default-stream work (the library's own stream is non-blocking and would
@@ -223,13 +260,10 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             float half_from = OA_HALF;
             if (const char* e = getenv("GMC_OA_HALF")) half_from = (float)atof(e);
             const EigSolve64 resolve = [&](double* A, double* w, int nb) -> int {
-                EigPlan<double>* p64 = nullptr;
-                GMC_TRY(st.eig_plan_for<double>(nd, nb, &p64));
-                GMC_TRY(st.resolve.reserve(p64->ws_bytes));
-                GMC_TRY(eig_bind<double>(p64, st.resolve.ptr));
-                GMC_TRY(syevd_batched<double>(s, *p64, A, w));
-                GMC_TRY(eig_check_error<double>(s, *p64));
-                return 0;
+                return st.resolve64(s, nd, A, w, nb);
+            };
+            const EigSolveAt cluster_eig = [&](double* A, double* w, int m, int nb, void* ws, size_t wsb) -> int {
+                return st.solve64_at(s, m, A, w, nb, ws, wsb);
             };
             EigPlan<float>* ep = nullptr;
             GMC_TRY(st.eig_plan_for<float>(nd, B, &ep));
@@ -239,7 +273,8 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             GMC_TRY(st.erpaws.reserve(align_up(mb) + outb));
             bind_out(align_up(mb));
             GMC_TRY(erpa_batched_mixed(s, *ep, nd, r.ab, alphas + a0, B, r.Gw, r.wdiag, st.erpaws.ptr, mb, out, n_min,
-                                           n_max, oa_extend, n_max > n_min ? &resolve : nullptr, half_from));
+                                           n_max, oa_extend, n_max > n_min ? &resolve : nullptr, half_from, nullptr,
+                                           &cluster_eig));
             // rare and as large as an FP64 eigensolver's workspace: not kept
             // beyond the call that needed it
             if (st.resolve.ptr) {

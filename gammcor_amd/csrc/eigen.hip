@@ -1318,6 +1318,23 @@ __global__ __launch_bounds__(256) void pad_tail_kernel(T* d, T* e, int n, int np
     for (int i = n - 1 + threadIdx.x; i < np - 1; i += 256) ee[i] = T(0);
 }
 
+// The reflectors of a padded reduction (strictly lower part of Ap, np x np)
+// into A (n x n): their pad rows are zero, so the back-transformation reads
+// them at order n.
+template <class T>
+__global__ void pad_vecs_kernel(const T* Ap, int np, T* A, int n)
+{
+    const int b = blockIdx.y;
+    const T* ap = Ap + (size_t)b * np * np;
+    T* a = A + (size_t)b * n * n;
+    const int64_t total = (int64_t)n * n;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % n), c = (int)(idx / n);
+        a[idx] = ap[r + (size_t)c * np];
+    }
+}
+
 // Eigenvectors and eigenvalues of the padded problems back to order n: the
 // first n eigenpairs are those of A, and their pad rows are zero.
 template <class T>
@@ -1383,12 +1400,11 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->bar, (B * G + 1) * BAR_STRIDE * sizeof(unsigned));
     take(p->err, 64);
     take(p->fro, B * sizeof(T));
-    if (p->np != p->n) {
-        take(p->Ap, B * nn * sizeof(T));
-        take(p->wp, B * n * sizeof(T));
-    } else {
-        p->Ap = p->wp = nullptr;
-    }
+    // padded problems: the reduction runs in Q (free until the divide &
+    // conquer), the eigenvalues of order np need their own vector
+    p->Ap = p->np != p->n ? p->Q : nullptr;
+    if (p->np != p->n) take(p->wp, B * n * sizeof(T));
+    else p->wp = nullptr;
     p->ws_bytes = align_up(off);
     return 0;
 }
@@ -1463,6 +1479,19 @@ static int eig_plan_host(int n, int batch, EigPlan<T>* p, std::vector<std::vecto
     GMC_TRY(eig_layout(p, nullptr));
     return 0;
 }
+
+template <class T>
+size_t eig_workspace_bytes(int n, int batch)
+{
+    EigPlan<T> p;
+    p.owns_tables = false;
+    std::vector<std::vector<int>> h_desc;
+    std::vector<unsigned char> h_split;
+    if (eig_plan_host(n, batch, &p, h_desc, h_split) != 0) return SIZE_MAX;
+    return p.ws_bytes;
+}
+template size_t eig_workspace_bytes<double>(int, int);
+template size_t eig_workspace_bytes<float>(int, int);
 
 template <class T>
 int eig_plan(int n, int batch, EigPlan<T>* p)
@@ -1659,7 +1688,11 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     }
     hipLaunchKernelGGL(sytrd_finish_kernel<T>, dim3((B + 255) / 256), dim3(256), 0, s, A, p.d, n, B);
     hipLaunchKernelGGL(clear_upper_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, A, n);
-    if (padded) hipLaunchKernelGGL(pad_tail_kernel<T>, dim3(B), dim3(256), 0, s, p.d, p.e, p.n, n);
+    if (padded) {
+        hipLaunchKernelGGL(pad_tail_kernel<T>, dim3(B), dim3(256), 0, s, p.d, p.e, p.n, n);
+        hipLaunchKernelGGL(pad_vecs_kernel<T>, dim3(grid1((int64_t)p.n * p.n), B), dim3(256), 0, s, A, n, Au, p.n);
+        A = Au;   // the reflectors at order p.n; Q is the divide & conquer's from here
+    }
     GMC_HIP(hipGetLastError());
     ps_tri.close();
     ProfScope ps_dc(s, f64 ? "eig_dc_f64" : "eig_dc_f32", 0.0);
@@ -1698,26 +1731,31 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         if (!capturing(s))
             GMC_HIP(hipFuncSetAttribute((const void*)larft_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds_t));
-        const int last = ((n - 2) / nbb) * nbb;
+        // H(0) ... H(nu-2) of order nu = p.n (a padded reduction's pad rows
+        // and reflectors are zero and identities) on the first nu columns of
+        // Z (ld n): V has ld nu, Z and the block products ld n
+        const int nu = p.n;
+        const int64_t nun = (int64_t)nu * nu;
+        const int last = ((nu - 2) / nbb) * nbb;
         for (int k = last; k >= 0; k -= nbb) {
-            const int kb = std::min(nbb, n - 1 - k);
-            const int m = n - k - 1;
-            const T* V = A + (k + 1) + (size_t)k * n;
+            const int kb = std::min(nbb, nu - 1 - k);
+            const int m = nu - k - 1;
+            const T* V = A + (k + 1) + (size_t)k * nu;
             T* Zs = p.Q + (k + 1);
-            GemmShape g1{kb, kb, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
+            GemmShape g1{kb, kb, m, true, false, (int64_t)nu, (int64_t)nu, (int64_t)nbb, nun, nun,
                          (int64_t)nbb * nbb, B, 0};
             GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gm));
             hipLaunchKernelGGL(larft_kernel<T>, dim3(B), dim3(256), lds_t, s, Gm, p.tau, Tm, n, k, kb, nbb);
             GMC_HIP(hipGetLastError());
-            GemmShape g2{kb, n, m, true, false, (int64_t)n, (int64_t)n, (int64_t)nbb, (int64_t)nn, (int64_t)nn,
+            GemmShape g2{kb, nu, m, true, false, (int64_t)nu, (int64_t)n, (int64_t)nbb, nun, (int64_t)nn,
                          (int64_t)nn, B, 0};
             // own_gemm: the library's MFMA tile (no library workspace)
             if (p.own_gemm) GMC_TRY(gemm<T>(s, g2, T(1), V, Zs, T(0), p.QK));
             else GMC_TRY(gemm_big<T>(s, g2, T(1), V, Zs, T(0), p.QK));
-            GemmShape g3{kb, n, kb, false, false, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb * nbb,
+            GemmShape g3{kb, nu, kb, false, false, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb, (int64_t)nbb * nbb,
                          (int64_t)nn, (int64_t)nn, B, 0};
             GMC_TRY(gemm<T>(s, g3, T(1), Tm, p.QK, T(0), p.QN));
-            GemmShape g4{m, n, kb, false, false, (int64_t)n, (int64_t)nbb, (int64_t)n, (int64_t)nn, (int64_t)nn,
+            GemmShape g4{m, nu, kb, false, false, (int64_t)nu, (int64_t)nbb, (int64_t)n, nun, (int64_t)nn,
                          (int64_t)nn, B, 0};
             if (p.own_gemm) GMC_TRY(gemm<T>(s, g4, T(-1), V, p.QN, T(1), Zs));
             else GMC_TRY(gemm_big<T>(s, g4, T(-1), V, p.QN, T(1), Zs));

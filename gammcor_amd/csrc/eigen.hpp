@@ -79,6 +79,11 @@ int eig_plan(int n, int batch, EigPlan<T>* p);
 template <class T>
 int eig_plan_at(int n, int batch, EigPlan<T>* p, void* mem, size_t mem_bytes, size_t* used, hipStream_t s);
 
+// Workspace bytes of a plan for (n, batch), without creating one (host
+// only); SIZE_MAX when no plan of that shape can be made.
+template <class T>
+size_t eig_workspace_bytes(int n, int batch);
+
 // Size the caller's scratch for eig_plan: p->ws_bytes after eig_plan.
 template <class T>
 int eig_bind(EigPlan<T>* p, void* ws);

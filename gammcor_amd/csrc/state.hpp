@@ -37,6 +37,14 @@ struct State {
     std::map<std::pair<int, int>, std::unique_ptr<EigPlan<float>>> eig32;
     template <class T>
     int eig_plan_for(int n, int batch, EigPlan<T>** out);
+    // FP64 eigenpairs of nb problems of order n (A: nb x n x n in, vectors
+    // out; w: nb x n) in `resolve`, in chunks whose workspace fits the free
+    // device memory (the mixed path's re-solves)
+    int resolve64(hipStream_t s, int n, double* A, double* w, int nb);
+    // The same for nb problems of order n in the caller's workspace ws
+    // (eig_workspace_bytes<double>(n, nb) bytes): the mixed path's large
+    // refinement clusters
+    int solve64_at(hipStream_t s, int n, double* A, double* w, int nb, void* ws, size_t ws_bytes);
     void release() {
         eig64.clear();
         eig32.clear();
