@@ -26,6 +26,65 @@ __device__ inline float wave_sum(float v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// The sums over the 64 lanes of a wave of KV values per lane (KV a power of
+// two, at most 64).  The values are first halved between lane pairs (a lane
+// keeps one half of its columns, adds its partner's share of them, and sends
+// the other half), so the KV sums take KV - 1 + 6 - log2(KV) exchanges
+// instead of 6 KV.  Returns the sum of column lane >> (6 - log2 KV): every
+// lane of each group of 64 / KV consecutive lanes holds its group's column.
+// The value of lane ^ O (O < 32 by ds_swizzle's xor mode, which takes the
+// pattern as an immediate and needs no address register).
+template <int O>
+__device__ inline float xor_lane(float v)
+{
+    if constexpr (O == 32) return __shfl_xor(v, 32, 64);
+    else return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (O << 10) | 0x1f));
+}
+template <int O>
+__device__ inline double xor_lane(double v)
+{
+    if constexpr (O == 32) {
+        return __shfl_xor(v, 32, 64);
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffff), (O << 10) | 0x1f);
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), (O << 10) | 0x1f);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+// One halving step (H values per lane kept, exchanged across lane bit O),
+// then the next; below one value per lane, plain xor sums.
+template <int H, int O, int KV, class T>
+__device__ inline T wave_sum_step(T (&s)[KV], int lane)
+{
+    if constexpr (H >= 1) {
+        const bool up = (lane & O) != 0;
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const T send = up ? s[k] : s[k + H];
+            const T keep = up ? s[k + H] : s[k];
+            s[k] = keep + xor_lane<O>(send);
+        }
+        return wave_sum_step<H / 2, O / 2>(s, lane);
+    } else if constexpr (O >= 1) {
+        s[0] += xor_lane<O>(s[0]);
+        return wave_sum_step<0, O / 2>(s, lane);
+    } else {
+        return s[0];
+    }
+}
+template <int KV, class T>
+__device__ inline T wave_sum_cols(T (&s)[KV], int lane)
+{
+    return wave_sum_step<KV / 2, 32>(s, lane);
+}
+template <int KV>
+constexpr int ilog2()
+{
+    if constexpr (KV <= 1) return 0;
+    else return 1 + ilog2<KV / 2>();
+}
+
 template <class T>
 __device__ inline T wave_prod(T v)
 {
@@ -483,11 +542,11 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
 #pragma unroll
                     for (int e = 0; e < VN; e++) scat<DET>(&u[c + e], w[e], dsc);
                 }
-#pragma unroll
-                for (int kk = 0; kk < KV; kk++) {
-                    const T t = wave_sum(s[kk]);
+                {
+                    const T t = wave_sum_cols<KV>(s, lane);
+                    const int kk = lane >> (6 - ilog2<KV>());
                     const int r = r0 + kk;
-                    if (lane == 0 && r > j && r < n) y[KV * qg + kk] = t;
+                    if ((lane & (64 / KV - 1)) == 0 && r > j && r < n) y[KV * qg + kk] = t;
                 }
             }
         } else
@@ -548,10 +607,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                     if (c > j && c > r0) scat<DET>(&u[c], w, dsc);
                 }
             }
+            {
+                const T t = wave_sum_cols<KC>(s, lane);
+                const int kk = lane >> (6 - ilog2<KC>());
 #pragma unroll
-            for (int kk = 0; kk < KC; kk++) {
-                const T t = wave_sum(s[kk]);
-                if (lane == 0 && ok[kk]) y[KC * qg + kk] = t;
+                for (int k2 = 0; k2 < KC; k2++)
+                    if (k2 == kk && (lane & (64 / KC - 1)) == 0 && ok[k2]) y[KC * qg + k2] = t;
             }
         }
         __syncthreads();
