@@ -1808,14 +1808,15 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     double* dscale = ar.take<double>(2 * (size_t)B);
     GMC_CHECK(ar.fits(), "ERPA mixed: scratch layout overflow");
     const bool stats = getenv("GMC_OA_STATS") != nullptr;
-    auto report = [&](const char* what, int it) -> int {
+    // (the first nb problems of the batch: the ones the last step refined)
+    auto report = [&](const char* what, int it, int nb) -> int {
         if (!stats) return 0;
-        std::vector<unsigned> h(B);
-        GMC_HIP(hipMemcpyAsync(h.data(), op.npair, B * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        std::vector<unsigned> h(nb);
+        GMC_HIP(hipMemcpyAsync(h.data(), op.npair, nb * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipStreamSynchronize(st));
         unsigned tot = 0;
         for (unsigned v : h) tot += v;
-        std::vector<unsigned> em((size_t)B * OA_EMAX_SLOTS);
+        std::vector<unsigned> em((size_t)nb * OA_EMAX_SLOTS);
         GMC_HIP(hipMemcpyAsync(em.data(), op.emax, em.size() * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipStreamSynchronize(st));
         float mx = 0.f;
@@ -1824,17 +1825,17 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
             std::memcpy(&f, &u, 4);
             mx = std::max(mx, f);
         }
-        std::vector<int> nc(B), ci(4 * (size_t)B);
-        GMC_HIP(hipMemcpyAsync(nc.data(), op.ncl, B * sizeof(int), hipMemcpyDeviceToHost, st));
+        std::vector<int> nc(nb), ci(4 * (size_t)nb);
+        GMC_HIP(hipMemcpyAsync(nc.data(), op.ncl, nb * sizeof(int), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipMemcpyAsync(ci.data(), op.cinfo, ci.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipStreamSynchronize(st));
         int ctot = 0, big = 0;
-        for (int b = 0; b < B; b++) {
+        for (int b = 0; b < nb; b++) {
             ctot += nc[b];
             big = std::max(big, ci[4 * b]);
         }
         fprintf(stderr, "[gmc] OA %s step %d: %u pairs in %d clusters (largest %d) over %d problems, max |E| %.3e\n",
-                what, it, tot, ctot, big, B, mx);
+                what, it, tot, ctot, big, nb, mx);
         return 0;
     };
     // Largest correction of the last step, per problem (oa_effective)
@@ -1844,30 +1845,81 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     // leaves unconverged go to `left`.  Without one, a problem whose
     // corrections are not finite or whose close pairs overflowed the record
     // fails the call.
-    auto refine = [&](const char* what, int wmode, const double* A, double* lamv, double* axbuf,
+    //
+    // After the first n_min steps a problem whose correction is below
+    // OA_LEAVE has converged and leaves the batch: the problems
+    // still refined are moved to the front (their slots of A, X and lamv
+    // swapped with converged ones), so the next steps' products and kernels
+    // run on them alone; the original order is restored at the end.
+    auto refine = [&](const char* what, int wmode, double* A, double* lamv, double* axbuf,
                       std::vector<int>& left) -> int {
         left.clear();
+        std::vector<int> prob(B);   // prob[slot] = the problem held in that slot
+        for (int b = 0; b < B; b++) prob[b] = b;
+        int nact = B;
+        // swap the data of slots a and c (Xn, free between steps, as the temporary)
+        auto swap_slots = [&](int a, int c) -> int {
+            auto sw = [&](double* base, size_t len) -> int {
+                GMC_HIP(hipMemcpyAsync(Xn, base + (size_t)a * len, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+                GMC_HIP(hipMemcpyAsync(base + (size_t)a * len, base + (size_t)c * len, len * sizeof(double),
+                                       hipMemcpyDeviceToDevice, st));
+                GMC_HIP(hipMemcpyAsync(base + (size_t)c * len, Xn, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+                return 0;
+            };
+            GMC_TRY(sw(X, nn));
+            GMC_TRY(sw(A, nn));
+            GMC_TRY(sw(lamv, (size_t)n));
+            std::swap(prob[a], prob[c]);
+            return 0;
+        };
+        int rc = 0;
         for (int it = 0; it < n_max; it++) {
-            GMC_TRY(oa_refine(st, n, B, wmode, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32,
+            GMC_TRY(oa_refine(st, n, nact, wmode, A, X, Xn, lamv, axbuf, Sb, Gb, partial, npart, delta, op, X32, E32,
                               P32, half_from, dscale, pe, cluster_eig));
-            GMC_TRY(report(what, it));
+            // the converged problems' eigenvectors into the new X buffer
+            if (nact < B)
+                GMC_HIP(hipMemcpyAsync(X + (size_t)nact * nn, Xn + (size_t)nact * nn, (size_t)(B - nact) * nn *
+                                       sizeof(double), hipMemcpyDeviceToDevice, st));
+            GMC_TRY(report(what, it, nact));
             if (stats && atoi(getenv("GMC_OA_STATS")) >= 2) {
-                // every problem's correction of the step
+                // every refined problem's correction of the step
                 fprintf(stderr, "[gmc] OA %s step %d per problem:", what, it);
-                for (int b = 0; b < B; b++) fprintf(stderr, " %.1e", pe[b]);
+                for (int b = 0; b < nact; b++) fprintf(stderr, " %d:%.1e", prob[b], pe[b]);
                 fprintf(stderr, "\n");
             }
-            const OaDecision d = oa_decide(pe, it, n_min, n_max, OA_TOL, oa_extend, OA_GIVEUP, resolve != nullptr);
-            if (d.next == OaNext::Continue) continue;
-            if (d.next == OaNext::Resolve) left = d.left;
-            GMC_CHECK(d.next != OaNext::Fail,
-                      "ERPA mixed: refinement of %s left %d problem(s) (first %d) with non-finite corrections, more "
-                      "than %u close pairs or a cluster of more than %d columns after %d steps; GMC_MIXED solves such "
-                      "problems again in FP64", what, (int)d.left.size(), d.left.empty() ? -1 : d.left[0], op.cap,
-                      OA_CMAX, n_max);
+            const float leave = std::min(OA_LEAVE, oa_extend);
+            const OaDecision d =
+                oa_decide(pe, it, n_min, n_max, OA_TOL, oa_extend, OA_GIVEUP, resolve != nullptr, leave);
+            if (d.next == OaNext::Continue) {
+                if (it + 1 >= n_min) {
+                    // converged problems leave; the others move to the front
+                    int lo = 0, hi = nact - 1;
+                    while (true) {
+                        while (lo <= hi && !(pe[lo] < leave)) lo++;
+                        while (hi >= lo && pe[hi] < leave) hi--;
+                        if (lo >= hi) break;
+                        GMC_TRY(swap_slots(lo, hi));
+                        std::swap(pe[lo], pe[hi]);
+                    }
+                    nact = lo;
+                }
+                continue;
+            }
+            if (d.next == OaNext::Resolve)
+                for (int sl : d.left) left.push_back(prob[sl]);
+            if (d.next == OaNext::Fail) {
+                set_error("ERPA mixed: refinement of %s left %d problem(s) (first %d) with non-finite corrections, "
+                          "more than %u close pairs or a cluster of more than half its columns after %d steps; "
+                          "GMC_MIXED solves such problems again in FP64", what, (int)d.left.size(),
+                          d.left.empty() ? -1 : prob[d.left[0]], op.cap, n_max);
+                rc = -2;
+            }
             break;
         }
-        return 0;
+        // the original order
+        for (int sl = 0; sl < B; sl++)
+            while (prob[sl] != sl) GMC_TRY(swap_slots(sl, prob[sl]));
+        return rc;
     };
     // The listed problems solved again in FP64: their matrices side by side
     // in Xn (free after the refinement), eigenvectors and eigenvalues back

@@ -104,6 +104,11 @@ constexpr float OA_EXTEND = 5e-4f;
 // steps is not converging (NBasis=400: close eigenvalue clusters of M keep it
 // near 0.1 for six steps).
 constexpr float OA_GIVEUP = 5e-2f;
+// A problem leaves the refinement once its correction is below OA_LEAVE: one
+// under OA_EXTEND still takes the steps that bring it there (up to
+// OA_MAX_STEPS), since each costs only its own products once the converged
+// problems have left the batch.
+constexpr float OA_LEAVE = 1e-6f;
 
 // A refinement step whose largest correction is at least OA_HALF forms its
 // update X E from IEEE half inputs (FP32 accumulation): the half rounding,

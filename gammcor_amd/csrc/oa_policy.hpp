@@ -52,21 +52,25 @@ struct OaDecision {
 // After step `it` (0-based) with per-problem corrections pe (oa_effective):
 //   - steps 0 .. n_min-1 always run unless every correction is below tol;
 //   - after them the refinement goes on while some correction is at least
-//     extend, up to n_max steps;
+//     `leave` (at most `extend`; by default equal to it) and below giveup,
+//     up to n_max steps;
 //   - with a re-solve available (adaptive mode), the problems whose
-//     correction is not below extend when it stops go to `left`; it stops
-//     early once every such problem is at or above giveup (not converging);
+//     correction is not below extend when it stops go to `left` (a
+//     correction in [leave, extend) counts as converged); it stops early
+//     once every problem at or above leave is at or above giveup (not
+//     converging);
 //   - without one (a fixed step count), a problem left at +inf fails the call.
 // Every comparison is written so that +inf and NaN count as not converged.
 inline OaDecision oa_decide(const std::vector<float>& pe, int it, int n_min, int n_max, float tol, float extend,
-                            float giveup, bool have_resolve)
+                            float giveup, bool have_resolve, float leave = -1.f)
 {
+    if (!(leave >= 0.f) || leave > extend) leave = extend;
     OaDecision d;
-    bool all_below_tol = true, all_below_extend = true, any_slow = false, any_bad = false;
+    bool all_below_tol = true, all_below_leave = true, any_slow = false, any_bad = false;
     for (float e : pe) {
         if (!(e < tol)) all_below_tol = false;
-        if (!(e < extend)) all_below_extend = false;
-        if (e >= extend && e < giveup) any_slow = true;   // still converging
+        if (!(e < leave)) all_below_leave = false;
+        if (e >= leave && e < giveup) any_slow = true;   // still converging
         if (!(e <= 3.4028235e38f)) any_bad = true;
     }
     const bool last = it + 1 >= n_max;
@@ -88,14 +92,14 @@ inline OaDecision oa_decide(const std::vector<float>& pe, int it, int n_min, int
         }
         return d;
     }
-    if (all_below_extend) {
+    if (all_below_leave) {
         d.next = OaNext::Stop;
         return d;
     }
     if (last || !any_slow) {
         for (int b = 0; b < (int)pe.size(); b++)
             if (!(pe[b] < extend)) d.left.push_back(b);
-        d.next = OaNext::Resolve;
+        d.next = d.left.empty() ? OaNext::Stop : OaNext::Resolve;
         return d;
     }
     d.next = OaNext::Continue;

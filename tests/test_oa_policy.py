@@ -22,8 +22,9 @@ DRIVER = textwrap.dedent(r"""
         switch (n) { case OaNext::Continue: return "continue"; case OaNext::Stop: return "stop";
                      case OaNext::Resolve: return "resolve"; default: return "fail"; }
     }
-    static void show(const char* tag, std::vector<float> pe, int it, int nmin, int nmax, bool res) {
-        OaDecision d = oa_decide(pe, it, nmin, nmax, 1e-8f, 5e-4f, 5e-2f, res);
+    static void show(const char* tag, std::vector<float> pe, int it, int nmin, int nmax, bool res,
+                     float leave = -1.f) {
+        OaDecision d = oa_decide(pe, it, nmin, nmax, 1e-8f, 5e-4f, 5e-2f, res, leave);
         std::printf("%s %s", tag, nm(d.next));
         for (int b : d.left) std::printf(" %d", b);
         std::printf("\n");
@@ -46,6 +47,13 @@ DRIVER = textwrap.dedent(r"""
         show("last", {1e-4f, 1e-3f, 2e-5f}, 5, 3, 6, true);
         show("overflow_last", {1e-4f, oa_effective(1e-6f, 5000, 4096)}, 5, 3, 6, true);
         show("nan_mid", {1e-4f, oa_effective(NAN, 0, 4096), 1e-3f}, 3, 3, 6, true);
+        // leave threshold 1e-6 below extend: corrections in [1e-6, 5e-4) keep
+        // refining until n_max, then count as converged
+        show("leave_mid", {2e-5f, 1e-7f}, 2, 3, 6, true, 1e-6f);
+        show("leave_done", {5e-7f, 1e-7f}, 2, 3, 6, true, 1e-6f);
+        show("leave_last", {2e-5f, 1e-3f, 1e-7f}, 5, 3, 6, true, 1e-6f);
+        show("leave_last_ok", {2e-5f, 1e-7f}, 5, 3, 6, true, 1e-6f);
+        show("leave_giveup", {0.12f, 1e-7f}, 3, 3, 6, true, 1e-6f);
         // fixed count (no re-solve): n_min == n_max == 2
         show("fixed_mid", {1e-3f, INFINITY}, 0, 2, 2, false);
         show("fixed_end", {1e-3f, 1e-2f}, 1, 2, 2, false);
@@ -94,6 +102,17 @@ def test_adaptive_mode(results):
     # 0.12 and +inf are both past the give-up level: stop and re-solve them
     assert results["giveup"] == ["resolve", "1", "2"]
     assert results["last"] == ["resolve", "1"]
+
+
+def test_leave_threshold(results):
+    """Below extend but above leave: the refinement continues (only those
+    problems, in the path), and at n_max such a problem counts as converged;
+    only a correction at or above extend goes to FP64."""
+    assert results["leave_mid"] == ["continue"]
+    assert results["leave_done"] == ["stop"]
+    assert results["leave_last"] == ["resolve", "1"]
+    assert results["leave_last_ok"] == ["stop"]
+    assert results["leave_giveup"] == ["resolve", "0"]
 
 
 def test_overflow_and_nan_go_to_fp64(results):
