@@ -217,8 +217,13 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
             const EigSolveAt cluster_eig = [&](double* A, double* w, int m, int nb, void* ws, size_t wsb) -> int {
                 return st.solve64_at(s, m, A, w, nb, ws, wsb);
             };
+            // (GMC_OA_STEPS fixes the step count, as for the single-system
+            // path: diagnostics, no FP64 re-solves)
+            int n_min = 3, n_max = OA_MAX_STEPS;
+            if (const char* e = getenv("GMC_OA_STEPS")) n_min = n_max = std::max(0, atoi(e));
             GMC_TRY(erpa_batched_mixed(s, *ep, npad, none, d_al, nalpha, nullptr, nullptr, st.erpaws.ptr, mb, outs[g],
-                                       3, OA_MAX_STEPS, OA_EXTEND, &resolve, OA_HALF, &bt, &cluster_eig));
+                                       n_min, n_max, OA_EXTEND, n_max > n_min ? &resolve : nullptr, OA_HALF, &bt,
+                                       &cluster_eig));
             GMC_TRY(eig_check_error<float>(s, *ep));
         }
         return 0;
