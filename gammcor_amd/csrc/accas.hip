@@ -957,12 +957,13 @@ __global__ __launch_bounds__(256) void oa_pairs_kernel(const double* S, const do
 // Clusters of one problem from its recorded pairs (union-find in LDS, one
 // workgroup per problem; dynamic LDS 2 n ints).  Out: ccl[i] the cluster of
 // column i (-1 for none), the members of every cluster in ascending order
-// (cmem, from ctab[k].x, ctab[k].y of them) and, for one of at most OA_CMAX
-// columns, the offset ctab[k].z of its m x m rotation in qbuf; ncl = number
-// of clusters, cinfo = (largest cluster, 1 if one holds more than half the
-// columns, clusters above OA_CMAX, 0).
+// (cmem, from ctab[k].x, ctab[k].y of them) and, for one of at most cmax
+// (<= OA_CMAX) columns, the offset ctab[k].z of its m x m rotation in qbuf;
+// ncl = number of clusters, cinfo = (largest cluster, 1 if one holds more
+// than half the columns, clusters above cmax, 0).
 __global__ __launch_bounds__(256) void oa_cluster_kernel(const unsigned* npair, const int2* pairs, unsigned cap, int n,
-                                                         int* ccl, int* cmem, int4* ctab, int* ncl, int* cinfo)
+                                                         int* ccl, int* cmem, int4* ctab, int* ncl, int* cinfo,
+                                                         int cmax)
 {
     extern __shared__ int csh[];
     int* par = csh;
@@ -1011,9 +1012,9 @@ __global__ __launch_bounds__(256) void oa_cluster_kernel(const unsigned* npair, 
                 ctab[(size_t)b * n + nc] = make_int4(tot, v, qt, 0);
                 big = max(big, v);
                 over |= 2 * v > n;
-                nbig += v > OA_CMAX;
+                nbig += v > cmax;
                 tot += v;
-                qt += v <= OA_CMAX ? v * v : 0;
+                qt += v <= cmax ? v * v : 0;
                 par[r] = v = -(nc + 1);
                 root[r] = 0;
                 nc++;
@@ -1140,7 +1141,8 @@ __global__ __launch_bounds__(256) void oa_e_kernel(const double* S, const double
 // clusters of problem blockIdx.y.
 __global__ __launch_bounds__(256) void oa_cluster_solve_kernel(const double* S, const double* G, double* lam,
                                                                const double* delta, int n, const int* ncl,
-                                                               const int4* ctab, const int* cmem, double* qbuf)
+                                                               const int4* ctab, const int* cmem, double* qbuf,
+                                                               int cmax)
 {
     __shared__ double H[OA_CMAX][OA_CMAX + 1], Q[OA_CMAX][OA_CMAX + 1];
     __shared__ double rc[OA_CMAX / 2], rs[OA_CMAX / 2];
@@ -1153,7 +1155,7 @@ __global__ __launch_bounds__(256) void oa_cluster_solve_kernel(const double* S, 
     for (int k = blockIdx.x; k < ncl[b]; k += gridDim.x) {
         const int4 ct = ctab[(size_t)b * n + k];
         const int m = ct.y;
-        if (m > OA_CMAX) continue;
+        if (m > cmax) continue;
         __syncthreads();
         for (int u = tid; u < m; u += 256) idx[u] = cmem[(size_t)b * n + ct.x + u];
         __syncthreads();
@@ -1248,7 +1250,7 @@ __global__ __launch_bounds__(256) void oa_cluster_solve_kernel(const double* S, 
 // X_C <- X_C Q_C for every cluster of problem blockIdx.y, on the 64 rows
 // from 64 blockIdx.x (the cluster's columns staged in LDS, then written).
 __global__ __launch_bounds__(256) void oa_cluster_apply_kernel(double* X, int n, const int* ncl, const int4* ctab,
-                                                               const int* cmem, const double* qbuf)
+                                                               const int* cmem, const double* qbuf, int cmax)
 {
     __shared__ double xs[64][OA_CMAX + 1];
     __shared__ double qs[OA_CMAX * OA_CMAX];
@@ -1259,7 +1261,7 @@ __global__ __launch_bounds__(256) void oa_cluster_apply_kernel(double* X, int n,
     for (int k = 0; k < ncl[b]; k++) {
         const int4 ct = ctab[(size_t)b * n + k];
         const int m = ct.y;
-        if (m > OA_CMAX) continue;
+        if (m > cmax) continue;
         __syncthreads();
         for (int u = tid; u < m; u += 256) idx[u] = cmem[(size_t)b * n + ct.x + u];
         const double* qk = qbuf + (size_t)b * OA_CMAX * n + ct.z;
@@ -1358,12 +1360,13 @@ struct OaPairs {
     int* cmem;          // B x n: members of the clusters
     int4* ctab;         // B x n: (first member, size, rotation offset) per cluster
     int* ncl;           // B
-    int* cinfo;         // B x 4: largest cluster, one above half the columns, clusters above OA_CMAX
-    int4* bigl;         // B x OA_BIGMAX: the clusters above OA_CMAX of one eigensolver chunk
+    int* cinfo;         // B x 4: largest cluster, one above half the columns, clusters above cmax
+    int4* bigl;         // OA_BIGCHUNK: the clusters above cmax of one eigensolver chunk
     double* qbuf;       // B x OA_CMAX x n: the clusters' rotations
     unsigned* emax;     // B x OA_EMAX_SLOTS: float bits of max |E_ij| (i != j) of the last step
     unsigned* pmax;     // B x OA_EMAX_SLOTS: float bits of the largest predicted next correction
     unsigned cap = OA_MAXPAIRS;   // pairs recorded per problem (<= OA_MAXPAIRS)
+    int cmax = OA_CMAX;           // largest cluster the workgroup Jacobi takes (2 .. OA_CMAX)
 };
 
 // Xn = X + s_b P, s_b the scale of problem b = i / nn (1 without scales).
@@ -1438,7 +1441,7 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
         clds_set = true;
     }
     hipLaunchKernelGGL(oa_cluster_kernel, dim3(B), dim3(256), clds, st, op.npair, op.pairs, op.cap, n, op.ccl,
-                       op.cmem, op.ctab, op.ncl, op.cinfo);
+                       op.cmem, op.ctab, op.ncl, op.cinfo, op.cmax);
     hipLaunchKernelGGL(oa_e_kernel, gtri, dim3(256), 0, st, S, G, lam, delta, AX, n, op.ccl, op.emax, op.pmax, wmode);
     GMC_HIP(hipGetLastError());
     // each problem's largest off-diagonal correction and prediction
@@ -1453,7 +1456,7 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
     GMC_HIP(hipMemcpyAsync(ncl.data(), op.ncl, B * sizeof(int), hipMemcpyDeviceToHost, st));
     GMC_HIP(hipStreamSynchronize(st));
     pe.assign(B, 0.f);
-    // Clusters above OA_CMAX columns: (problem, cluster, size, first member),
+    // Clusters above cmax columns: (problem, cluster, size, first member),
     // largest first, in eigensolver chunks of one padded order that fit the
     // old X's space (free once the update has swapped X and Xn).  A problem
     // whose cluster cannot be solved so stays unconverged.
@@ -1475,7 +1478,7 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
         std::vector<int4> tab(ncl[b]);
         GMC_HIP(hipMemcpy(tab.data(), op.ctab + (size_t)b * n, tab.size() * sizeof(int4), hipMemcpyDeviceToHost));
         for (int k = 0; k < ncl[b]; k++)
-            if (tab[k].y > OA_CMAX) bigs.push_back(make_int4(b, k, tab[k].y, tab[k].x));
+            if (tab[k].y > op.cmax) bigs.push_back(make_int4(b, k, tab[k].y, tab[k].x));
     }
     std::stable_sort(bigs.begin(), bigs.end(), [](const int4& u, const int4& v) { return u.z > v.z; });
     struct BigChunk {
@@ -1567,12 +1570,12 @@ static int oa_refine(hipStream_t st, int n, int B, int wmode, const double* A, d
     // the clusters' rotations (from S, G and the step's eigenvalues) and
     // eigenvalue estimates, applied after the update
     hipLaunchKernelGGL(oa_cluster_solve_kernel, dim3(64, B), dim3(256), 0, st, S, G, lam, delta, n, op.ncl, op.ctab,
-                       op.cmem, op.qbuf);
+                       op.cmem, op.qbuf, op.cmax);
     hipLaunchKernelGGL(oa_add_kernel, dim3(gcopy), dim3(256), 0, st, X, P32, Xn, cnt, snn, s_b);
     GMC_HIP(hipGetLastError());
     std::swap(X, Xn);
     hipLaunchKernelGGL(oa_cluster_apply_kernel, dim3((n + 63) / 64, B), dim3(256), 0, st, X, n, op.ncl, op.ctab,
-                       op.cmem, op.qbuf);
+                       op.cmem, op.qbuf, op.cmax);
     GMC_HIP(hipGetLastError());
     // the large clusters: H in the old X's space, its eigenvectors rotate
     // the cluster's columns (moved through AX, free after the update)
@@ -1802,6 +1805,9 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
     // GMC_OA_MAXPAIRS lowers the number of close pairs recorded per problem
     // (diagnostics: forces the over-the-cap path)
     if (const char* e = getenv("GMC_OA_MAXPAIRS")) op.cap = (unsigned)std::min(std::max(0, atoi(e)), OA_MAXPAIRS);
+    // GMC_OA_CMAX lowers the largest cluster the workgroup Jacobi solves
+    // (diagnostics: sends small clusters to the batched eigensolver)
+    if (const char* e = getenv("GMC_OA_CMAX")) op.cmax = std::min(std::max(2, atoi(e)), OA_CMAX);
     double* wre = ar.take<double>(B * n);
     char* pwork = ar.take<char>(potrf_work_bytes(n, B));
     int* info = ar.take<int>(B);
@@ -1829,13 +1835,15 @@ int erpa_batched_mixed(hipStream_t st, EigPlan<float>& ep, int n, const AbPair& 
         GMC_HIP(hipMemcpyAsync(nc.data(), op.ncl, nb * sizeof(int), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipMemcpyAsync(ci.data(), op.cinfo, ci.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         GMC_HIP(hipStreamSynchronize(st));
-        int ctot = 0, big = 0;
+        int ctot = 0, big = 0, nlarge = 0;
         for (int b = 0; b < nb; b++) {
             ctot += nc[b];
             big = std::max(big, ci[4 * b]);
+            nlarge += ci[4 * b + 2];
         }
-        fprintf(stderr, "[gmc] OA %s step %d: %u pairs in %d clusters (largest %d) over %d problems, max |E| %.3e\n",
-                what, it, tot, ctot, big, nb, mx);
+        fprintf(stderr,
+                "[gmc] OA %s step %d: %u pairs in %d clusters (largest %d, %d to the eigensolver) over %d problems, "
+                "max |E| %.3e\n", what, it, tot, ctot, big, nlarge, nb, mx);
         return 0;
     };
     // Largest correction of the last step, per problem (oa_effective)

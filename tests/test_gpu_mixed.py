@@ -117,6 +117,39 @@ def _n48_system():
     return host.DeviceSystem(tw, s.occ, s.xone, s.gamma_act, 48, 6, 4, s.nele), a, w
 
 
+def test_clusters_through_the_batched_eigensolver(cuda, monkeypatch, capfd):
+    """The refinement's clusters above the workgroup Jacobi's size go to the
+    batched FP64 eigensolver (N=400's first step has one of about 1 900
+    columns per problem).  GMC_OA_CMAX=2 sends every cluster of three or more
+    columns there: the result stays within 1 uEh of the FP64 path, and
+    within 1e-9 Eh of the default run, whose clusters the Jacobi solves."""
+    import re
+    from gammcor_amd import host, synth, _lib
+    s = synth.config_system("n160_cas88", with_integrals=False)
+    import torch
+    ao = synth.ao_integrals_device(s.chol, cuda)
+    ure = torch.from_numpy(np.asfortranarray(s.uaomo).ravel(order="F")).to(cuda)
+    _lib.call("gmc_twono1_dev", ao.data_ptr(), ure.data_ptr(), s.nbasis, ao.numel(), None)
+    d = host.DeviceSystem(ao, s.occ, s.xone, s.gamma_act, s.nbasis, s.ninact, s.nact, s.nele, device=cuda)
+    a, w = host.GauLeg(0.0, 1.0, 8)
+    ref = host.ACECORR(d, a, w, prec=host.GMC_FP64)
+    monkeypatch.setenv("GMC_OA_STATS", "1")
+    capfd.readouterr()
+    m0 = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+    err0 = capfd.readouterr().err
+    monkeypatch.setenv("GMC_OA_CMAX", "2")
+    m1 = host.ACECORR(d, a, w, prec=host.GMC_MIXED)
+    err1 = capfd.readouterr().err
+    print(err0, err1, flush=True)
+    big0 = [int(x) for x in re.findall(r"OA M step \d+: \d+ pairs in \d+ clusters \(largest \d+, (\d+) to", err0)]
+    big1 = [int(x) for x in re.findall(r"OA M step \d+: \d+ pairs in \d+ clusters \(largest \d+, (\d+) to", err1)]
+    largest = [int(x) for x in re.findall(r"OA M step \d+: \d+ pairs in \d+ clusters \(largest (\d+)", err1)]
+    assert big0 and max(largest) >= 3 and sum(big1) > sum(big0), (largest, big0, big1)
+    assert "solved again in FP64" not in err1
+    assert abs(m1["ecorr"] - ref["ecorr"]) < 1e-6
+    assert abs(m1["ecorr"] - m0["ecorr"]) < 1e-9
+
+
 def test_half_precision_updates_on_every_step(cuda, monkeypatch, capfd):
     """GMC_OA_HALF at 1e-12 forms every refinement update from plain half
     inputs (the branch larger systems take for their first corrections):
