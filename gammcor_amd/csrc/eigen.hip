@@ -112,6 +112,28 @@ __device__ T block_sum(T v, T* red)
     for (int w = 0; w < nw; w++) s += red[w];
     return s;
 }
+// Two such sums at once, over one pair of barriers (red: 2 x the waves);
+// each comes out as block_sum would give it.
+template <class T>
+__device__ void block_sum2(T& a, T& b, T* red)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        red[wave] = a;
+        red[nw + wave] = b;
+    }
+    __syncthreads();
+    T sa = 0, sb = 0;
+    for (int w = 0; w < nw; w++) {
+        sa += red[w];
+        sb += red[nw + w];
+    }
+    a = sa;
+    b = sb;
+}
 template <class T>
 __device__ T block_max(T v, T* red)
 {
@@ -347,8 +369,8 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
     T* Wr = Ar + nb;                                // W'(j+1, c), nb
     T* a2 = Wr + nb;
     T* t12 = a2 + nb;                               // 2 nb
-    T* red = t12 + 2 * nb;                          // SY_WAVES
-    T* sc = red + SY_WAVES;                         // [0]=tau [1]=scal [2]=fixed-point scale (DET)
+    T* red = t12 + 2 * nb;                          // 2 SY_WAVES
+    T* sc = red + 2 * SY_WAVES;                     // [0]=tau [1]=scal [2]=fixed-point scale (DET)
 
     // panel entries of an owned row (slot q, row r): LDS copies or the global panel
     auto vpan = [&](int c, int q, int r) -> T {
@@ -405,7 +427,9 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                     if (c < n) v[c] = tmp[e];
                 }
             }
-            const T xn2 = block_sum(xpart, red);
+            // the G norm partials: one wave's sum when they fit in it (the
+            // same value block_sum gives, without its two barriers)
+            const T xn2 = G <= 64 ? (wave == 0 ? wave_sum(xpart) : T(0)) : block_sum(xpart, red);
             if (tid == 0) {
                 const T alpha = *p_alpha;
                 T tau, beta, scal;
@@ -621,6 +645,12 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
             for (int c = j + 1 + tid; c < n; c += SY_THREADS) u[c] = det_value(&u[c], dsc);
             __syncthreads();
         }
+        // row j+1 of the V and W' panels (final since the last column's
+        // barriers), loaded ahead of barrier 2
+        for (int c = tid; c < i; c += SY_THREADS) {
+            Ar[c] = A[j + 1 + (size_t)(k + c) * n];
+            Wr[c] = W[j + 1 + (size_t)c * n];
+        }
         {
             // v^T y partial of this workgroup: own rows' dot parts and its scatter partial
             T vy = 0;
@@ -669,10 +699,6 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                     lds_add(&t12[c], acc);
                 }
             }
-            for (int c = tid; c < i; c += SY_THREADS) {
-                Ar[c] = A[j + 1 + (size_t)(k + c) * n];
-                Wr[c] = W[j + 1 + (size_t)c * n];
-            }
             __syncthreads();
             // a2(i) = -tau/2 w^T v with w^T v = tau (v^T y - 2 t1.t2), and
             // W'(j+1, i) = tau y(j+1) + a2(i) v(j+1), v(j+1) = 1
@@ -681,8 +707,8 @@ __global__ __launch_bounds__(SY_THREADS) void sytrd_panel_kernel(PanelArgs<T> g)
                 x -= T(2) * t12[tid] * t12[i + tid];
                 yr -= Ar[tid] * t12[tid] + Wr[tid] * t12[i + tid];
             }
-            const T wv_over_tau = block_sum(x, red);
-            const T ysum = block_sum(yr, red);
+            block_sum2(x, yr, red);
+            const T wv_over_tau = x, ysum = yr;
             if (tid == 0) {
                 const T a2i = T(-0.5) * tau * (tau * wv_over_tau);
                 a2[i] = a2i;
@@ -1696,7 +1722,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         // fit), y and the small arrays, and the own rows of the V and W'
         // panels when they fit beside them
         constexpr size_t LDS_MAX = 160 * 1024;
-        const size_t small = (size_t)(nslots_max + 5 * nb + SY_WAVES + 8) * sizeof(T);
+        const size_t small = (size_t)(nslots_max + 5 * nb + 2 * SY_WAVES + 8) * sizeof(T);
         const size_t panels = 2 * (size_t)nb * nslots_max * sizeof(T);
         const size_t vec_n = (size_t)n * sizeof(T);
         const bool vlds = 2 * vec_n + small <= LDS_MAX;
