@@ -13,7 +13,9 @@
 // a batch, t(npad, B) = c0 + c1 npad + c2 B npad^2 + c3 B npad^3 (c from the autotuner,
 // gammcor_amd/tune_batch.py), over contiguous runs of the sorted systems.
 // With `graph` the FP64 group sequence is captured into one HIP graph after
-// an eager pass (which sizes every workspace) and replayed from it.
+// an eager pass (which sizes every workspace) and replayed from it; the graph
+// is kept and replayed alone by later calls on the same workload while no
+// workspace has been reallocated.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -229,38 +231,71 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
         return 0;
     };
     if (graph) {
-        // eager pass on the library's own GEMMs (what the graph captures),
-        // then the same sequence captured on the library's stream and replayed
-        const bool was = set_own_gemm(true);
-        int rc = 0;
-        for (size_t g = 0; g < groups.size() && !rc; g++) rc = run_group(g, true);
-        if (!rc) {
-            hipStream_t cs = st.stream;
-            rc = hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
-            hipGraph_t gr = nullptr;
-            hipGraphExec_t ex = nullptr;
-            if (!rc && hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) rc = -2;
-            if (!rc) {
-                for (size_t g = 0; g < groups.size() && !rc; g++) {
-                    hipStream_t keep = s;
-                    s = cs;
-                    rc = run_group(g, false);
-                    s = keep;
-                }
-                const hipError_t e = hipStreamEndCapture(cs, &gr);
-                if (!rc && e != hipSuccess) rc = -3;
-            }
-            if (!rc && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) rc = -4;
-            if (!rc && hipGraphLaunch(ex, cs) != hipSuccess) rc = -5;
-            if (!rc && hipStreamSynchronize(cs) != hipSuccess) rc = -6;
-            if (ex) (void)hipGraphExecDestroy(ex);
-            if (gr) (void)hipGraphDestroy(gr);
-            if (rc < -1) {
-                (void)hipGetLastError();
-                set_error("ACECORR batch: HIP graph capture or replay failed (step %d)", -rc);
-            }
+        // The group sequence of this workload (sizes, alphas, grouping and
+        // the batch workspace's address) as a graph on the library's stream:
+        // replayed as it is when captured before and no workspace has moved
+        // since, else an eager pass on the library's own GEMMs (which sizes
+        // every workspace), then the same sequence captured and kept.
+        std::vector<double> key{(double)nsys, (double)nalpha, (double)(uintptr_t)st.batch.ptr};
+        key.insert(key.end(), alphas, alphas + nalpha);
+        for (int k = 0; k < nsys; k++) {
+            const int i = order[k];
+            key.insert(key.end(), {(double)ndimx[i], (double)nbasis[i], (double)ninact[i], (double)nact[i]});
         }
-        set_own_gemm(was);
+        for (const auto& gp : groups) {
+            key.push_back(gp.begin);
+            key.push_back(gp.end);
+            key.push_back(gp.npad);
+        }
+        hipStream_t cs = st.stream;
+        int rc = 0;
+        const bool hit = st.bgraph.ex && st.bgraph.key == key && st.bgraph.gen == workspace_generation();
+        if (hit) {
+            rc = hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
+            if (!rc && hipGraphLaunch(st.bgraph.ex, cs) != hipSuccess) rc = -5;
+            if (!rc && hipStreamSynchronize(cs) != hipSuccess) rc = -6;
+        } else {
+            st.drop_batch_graph();
+            const bool was = set_own_gemm(true);
+            for (size_t g = 0; g < groups.size() && !rc; g++) rc = run_group(g, true);
+            if (!rc) {
+                rc = hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
+                const uint64_t gen = workspace_generation();
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ex = nullptr;
+                if (!rc && hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) rc = -2;
+                if (!rc) {
+                    for (size_t g = 0; g < groups.size() && !rc; g++) {
+                        hipStream_t keep = s;
+                        s = cs;
+                        rc = run_group(g, false);
+                        s = keep;
+                    }
+                    const hipError_t e = hipStreamEndCapture(cs, &gr);
+                    if (!rc && e != hipSuccess) rc = -3;
+                }
+                // the capture met every size the eager pass reserved: nothing moved
+                if (!rc && workspace_generation() != gen) rc = -7;
+                if (!rc && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) rc = -4;
+                if (!rc && hipGraphLaunch(ex, cs) != hipSuccess) rc = -5;
+                if (!rc && hipStreamSynchronize(cs) != hipSuccess) rc = -6;
+                if (!rc) {
+                    st.bgraph.gr = gr;
+                    st.bgraph.ex = ex;
+                    st.bgraph.key = key;
+                    st.bgraph.gen = gen;
+                } else {
+                    if (ex) (void)hipGraphExecDestroy(ex);
+                    if (gr) (void)hipGraphDestroy(gr);
+                }
+            }
+            set_own_gemm(was);
+        }
+        if (rc < -1) {
+            (void)hipGetLastError();
+            st.drop_batch_graph();
+            set_error("ACECORR batch: HIP graph capture or replay failed (step %d)", -rc);
+        }
         if (rc) return rc;
         // the panel kernel's barrier flag of every replayed group
         std::vector<int> fl(groups.size());

@@ -30,9 +30,16 @@ void set_error(const char* fmt, ...)
 
 const char* last_error() { return g_err; }
 
+namespace {
+uint64_t g_ws_gen = 0;
+}
+uint64_t workspace_generation() { return g_ws_gen; }
+void bump_workspace_generation() { g_ws_gen++; }
+
 int Workspace::reserve(size_t need)
 {
     if (need <= bytes) return 0;
+    g_ws_gen++;
     if (ptr) GMC_HIP(hipFree(ptr));
     ptr = nullptr;
     bytes = 0;
@@ -279,6 +286,7 @@ int ac_alpha_batches(hipStream_t s, AcRun& r, int prec, int cap, int nalpha, con
             // beyond the call that needed it
             if (st.resolve.ptr) {
                 GMC_HIP(hipStreamSynchronize(s));
+                bump_workspace_generation();
                 GMC_HIP(hipFree(st.resolve.ptr));
                 st.resolve = Workspace{};
             }

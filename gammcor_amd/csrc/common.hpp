@@ -41,6 +41,12 @@ struct Status {
     } while (0)
 
 // Device scratch owned by the library: grows on demand, freed at finalize.
+// Counts the reallocations of every Workspace (and other frees of device
+// memory a captured HIP graph could read): a graph captured at one count is
+// replayed only while the count is unchanged.
+uint64_t workspace_generation();
+void bump_workspace_generation();
+
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;

@@ -4,6 +4,7 @@
 #include <map>
 #include <memory>
 #include <tuple>
+#include <vector>
 
 #include "common.hpp"
 #include "eigen.hpp"
@@ -45,7 +46,23 @@ struct State {
     // (eig_workspace_bytes<double>(n, nb) bytes): the mixed path's large
     // refinement clusters
     int solve64_at(hipStream_t s, int n, double* A, double* w, int nb, void* ws, size_t ws_bytes);
+    // The batching form's last captured FP64 group sequence: replayed while
+    // its key (sizes, alphas, groups, workspace address) and the workspace
+    // generation are unchanged.
+    struct BatchGraph {
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ex = nullptr;
+        std::vector<double> key;
+        uint64_t gen = 0;
+    } bgraph;
+    void drop_batch_graph() {
+        if (bgraph.ex) (void)hipGraphExecDestroy(bgraph.ex);
+        if (bgraph.gr) (void)hipGraphDestroy(bgraph.gr);
+        bgraph = BatchGraph{};
+    }
     void release() {
+        drop_batch_graph();
+        bump_workspace_generation();
         eig64.clear();
         eig32.clear();
         blas_release(blas);
