@@ -206,6 +206,12 @@ def bench_batch(args):
     t_gr, r_gr = timed(lambda: host.ACECORR_BATCH(systems, a, w, prec=host.GMC_FP64, graph=True), 1, 1)
     t_one, e_one = timed(lambda: np.array([host.ACECORR(d, a, w, prec=host.GMC_FP64)["ecorr"] for d in systems]), 1, 1)
     grp, npad = host.batch_plan([d.ndimx for d in systems], nal)
+    # GMC_MIXED on the batching form: groups of padded order >= 3000 mixed,
+    # the others FP64 (GMC_BATCH_MIXED_MIN moves the threshold)
+    mixed_min = int(os.environ.get("GMC_BATCH_MIXED_MIN", "3000"))
+    n_mixed = len({int(g) for g, p in zip(grp, npad) if p >= mixed_min})
+    dtype_batch = ("f32 eigen solves + f64 (Ogita-Aishima refinement, assembly, quadrature) on groups of padded "
+                   f"order >= {mixed_min}, f64 below ({n_mixed} of {int(grp.max() + 1)} groups mixed)")
     line = {
         "metric": "AC-CAS solves/s, batching workload: 64 systems (NBasis 40-140) x 16 alpha points",
         "value": nsys / t_mix,
@@ -217,7 +223,7 @@ def bench_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 eigen solves + f64 (Ogita-Aishima refinement, assembly, quadrature)",
+        "dtype": dtype_batch,
         "data": "synthetic (seeded Cholesky-factor AO integrals, QR orbitals, random-CI RDMs)",
         "config": {"workload": "batch64: 64 systems NBasis uniform in [40,140] (seed 3), CAS(4,4)/CAS(6,6) alternating, "
                                "16 Gauss-Legendre alpha points, all in the batching form",

@@ -30,6 +30,13 @@
 
 namespace gmc {
 
+// Padded order from which a batching-form group takes the mixed path: on the
+// 64-system workload (padded orders 432-2144, 16 alpha points, up to 256
+// problems per group) every group ran faster in FP64 (20.7 against 19.1
+// solves/s), and single systems gain from order ~3000 on (NBasis=160,
+// NDimX=2664: 0.99x; NBasis=240, NDimX=4524: 1.11x).
+constexpr int BATCH_MIXED_MIN = 3000;
+
 namespace {
 
 // Measured on one MI355X (FP64 batches of 16-256 problems, npad 256-1928,
@@ -184,11 +191,18 @@ int gmc_acecorr_batch_dev(int prec, int nsys, const int* nbasis, const int* nina
     const AbPair none{};
     // one group: its plans and workspaces sized (allocating when they grow),
     // then its solve; the replayed pass runs it with every size already met
+    // GMC_MIXED runs the mixed path on the groups of padded order at least
+    // BATCH_MIXED_MIN and FP64 on the others, where the FP32 eigensolves save
+    // less than the refinement's FP64 products cost (GMC_BATCH_MIXED_MIN
+    // moves the threshold; 0 runs every group mixed)
+    int mixed_min = BATCH_MIXED_MIN;
+    if (const char* e = getenv("GMC_BATCH_MIXED_MIN")) mixed_min = std::max(0, atoi(e));
     auto run_group = [&](size_t g, bool check) -> int {
         const int npad = groups[g].npad, ng = groups[g].end - groups[g].begin, B = ng * nalpha;
         ErpaBatchTables bt{d_tab + groups[g].begin, ng};
-        ProfScope ps(s, prec == GMC_FP64 ? "batch_group_f64" : "batch_group_mixed", (double)B);
-        if (prec == GMC_FP64) {
+        const bool f64 = prec == GMC_FP64 || npad < mixed_min;
+        ProfScope ps(s, f64 ? "batch_group_f64" : "batch_group_mixed", (double)B);
+        if (f64) {
             EigPlan<double>* ep = nullptr;
             GMC_TRY(st.eig_plan_for<double>(npad, B, &ep));
             GMC_TRY(st.scratch.reserve(ep->ws_bytes));

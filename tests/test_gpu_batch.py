@@ -95,8 +95,11 @@ def test_batch_form_fp64_matches_per_system(workload):
     assert np.all(de < 1e-9), (np.argmax(de), de.max())
 
 
-def test_batch_form_mixed_within_1ueh(workload):
+def test_batch_form_mixed_within_1ueh(workload, monkeypatch):
+    """Every group on the mixed path (GMC_BATCH_MIXED_MIN=0: by default
+    groups below order 3000, all of this workload's, run in FP64)."""
     from gammcor_amd import host
+    monkeypatch.setenv("GMC_BATCH_MIXED_MIN", "0")
     t0 = time.perf_counter()
     r = host.ACECORR_BATCH(workload["systems"], workload["alphas"], workload["weights"], prec=host.GMC_MIXED)
     print(f"batch form mixed: {time.perf_counter() - t0:.2f} s", flush=True)
@@ -113,7 +116,11 @@ def test_batch_form_graph_replays(workload):
     eager = host.ACECORR_BATCH(*args, prec=host.GMC_FP64)["ecorr"]
     for rep in range(4):
         if rep == 2:
-            host.ACECORR_BATCH(*args, prec=host.GMC_MIXED)
+            os.environ["GMC_BATCH_MIXED_MIN"] = "0"
+            try:
+                host.ACECORR_BATCH(*args, prec=host.GMC_MIXED)
+            finally:
+                del os.environ["GMC_BATCH_MIXED_MIN"]
         t0 = time.perf_counter()
         g = host.ACECORR_BATCH(*args, prec=host.GMC_FP64, graph=True)["ecorr"]
         print(f"graph call {rep}: {time.perf_counter() - t0:.2f} s (capture and replay, or replay alone), "
