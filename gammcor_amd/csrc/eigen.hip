@@ -927,6 +927,12 @@ struct DcBufs {
     T *lam, *Q, *QK, *QN, *UB;
     T *dsort, *zsort, *dK, *zK, *taus, *zh, *EV, *rotc, *rots;
     int *perm, *lst, *lstD, *org, *rank, *rotp, *rotj, *kcnt, *rcnt;
+    // Column grouping of the merge product: a non-deflated column t of a
+    // merge comes from the first child (rows s..s+m1 only), from both (a
+    // deflation rotation mixed one of each) or from the second (rows
+    // s+m1..s+m only); kpos[t] is its place with the first kind first and the
+    // second last, kc1 / kc2 (per merge) the counts of the first two kinds.
+    int *kpos, *kc1, *kc2;
 };
 
 __device__ inline int find_merge(const int* desc, int nmerge, int x)
@@ -951,6 +957,7 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
 {
     __shared__ T red[8];
     __shared__ T cd[256], cz[256];
+    __shared__ int cp[256];
     const int n = a.n;
     const int p = blockIdx.x / a.nmerge, q = blockIdx.x % a.nmerge;
     const int s = a.desc[4 * q], m1 = a.desc[4 * q + 1], m2 = a.desc[4 * q + 2];
@@ -997,9 +1004,12 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
     const T tol = T(8) * Eps<T>::v * fmax(dmax, zmax);
     __syncthreads();
 
-    // sequential deflation scan (thread 0), m values staged 256 at a time
-    int k = 0, ndf = 0, nrot = 0, pj = -1;
+    // sequential deflation scan (thread 0), m values staged 256 at a time;
+    // tpj is the kind of the pending column pj (1: first child, 3: second,
+    // 2: mixed by a rotation), kept in kpos until the grouping below
+    int k = 0, ndf = 0, nrot = 0, pj = -1, tpj = 0;
     T dpj = 0, zpj = 0;
+    int* kpos = b.kpos + po + s;
     int* lst = b.lst + po + s;
     int* lstD = b.lstD + po + s;
     T* dK = b.dK + po + s;
@@ -1009,6 +1019,7 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
         if (c0 + (int)threadIdx.x < m) {
             cd[threadIdx.x] = dsort[c0 + threadIdx.x];
             cz[threadIdx.x] = zsort[c0 + threadIdx.x];
+            cp[threadIdx.x] = perm[c0 + threadIdx.x] < m1 ? 1 : 3;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1016,6 +1027,7 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
             for (int jj = 0; jj < ce; jj++) {
                 const int j = c0 + jj;
                 T dj = cd[jj], zj = cz[jj];
+                const int tj = cp[jj];
                 if (rho * fabs(zj) <= tol) {
                     lstD[ndf++] = j;
                     continue;
@@ -1024,6 +1036,7 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
                     pj = j;
                     dpj = dj;
                     zpj = zj;
+                    tpj = tj;
                     continue;
                 }
                 T sv = zpj, cv = zj;
@@ -1044,15 +1057,18 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
                     pj = j;
                     dpj = djn;
                     zpj = tv;
+                    tpj = tpj == tj ? tj : 2;   // the rotation mixes the two columns
                     dsort[j] = djn;
                 } else {
                     lst[k] = pj;
                     dK[k] = dpj;
                     zK[k] = zpj;
+                    kpos[k] = tpj;
                     k++;
                     pj = j;
                     dpj = dj;
                     zpj = zj;
+                    tpj = tj;
                 }
             }
         }
@@ -1062,10 +1078,24 @@ __global__ __launch_bounds__(256) void dc_prep_kernel(DcArgs a, DcBufs<T> b)
             lst[k] = pj;
             dK[k] = dpj;
             zK[k] = zpj;
+            kpos[k] = tpj;
             k++;
         }
         b.kcnt[po + q] = k;
         b.rcnt[po + q] = nrot;
+        // the grouping: first-child columns, then mixed, then second-child
+        int n1 = 0, n2 = 0;
+        for (int t = 0; t < k; t++) {
+            n1 += kpos[t] == 1;
+            n2 += kpos[t] == 2;
+        }
+        int i1 = 0, i2 = n1, i3 = n1 + n2;
+        for (int t = 0; t < k; t++) {
+            const int kind = kpos[t];
+            kpos[t] = kind == 1 ? i1++ : (kind == 2 ? i2++ : i3++);
+        }
+        b.kc1[po + q] = n1;
+        b.kc2[po + q] = n2;
     }
 }
 
@@ -1266,16 +1296,22 @@ __global__ __launch_bounds__(256) void dc_vec_kernel(DcArgs a, DcBufs<T> b)
     }
     ss = wave_sum(ss);
     const T inv = T(1) / sqrt(ss);
+    // U's rows and QK's columns in the grouped order (kpos): their product is
+    // the same, and the merge GEMM skips the zero blocks
+    const int* kpos = b.kpos + po + s;
     T* U = b.UB + po * n + (size_t)(s + i) * n;
-    for (int t = lane; t < k; t += 64) U[t] = zh[t] / ((dK[t] - dorg) - ti) * inv;
-    // gather: QK[:, s+i] = Q[s.., s + perm[lst[i]]]
+    for (int t = lane; t < k; t += 64) U[kpos[t]] = zh[t] / ((dK[t] - dorg) - ti) * inv;
+    // gather: QK[:, s+kpos[i]] = Q[s.., s + perm[lst[i]]]
     const int col = s + b.perm[po + s + b.lst[po + s + i]];
     const T* src = b.Q + po * n + s + (size_t)col * n;
-    T* dst = b.QK + po * n + (size_t)(s + i) * n;
+    T* dst = b.QK + po * n + (size_t)(s + kpos[i]) * n;
     for (int r = lane; r < m; r += 64) dst[r] = src[r];
 }
 
-// Grouped merge GEMM: QN[:, s..s+k) = QK[:, s..s+k) * U, per merge.
+// Grouped merge GEMM: QN[:, s..s+k) = QK[:, s..s+k) * U, per merge, as two
+// products that skip QK's zero blocks (columns grouped by kpos): the first
+// child's m1 rows over the first-child and mixed columns, the second child's
+// m2 rows over the mixed and second-child columns.
 template <class T>
 __global__ __launch_bounds__(256) void dc_gemm_kernel(DcArgs a, DcBufs<T> b, int64_t tiles)
 {
@@ -1288,16 +1324,34 @@ __global__ __launch_bounds__(256) void dc_gemm_kernel(DcArgs a, DcBufs<T> b, int
         if (a.desc[4 * mid + 3] <= t) { q = mid; lo = mid + 1; } else hi = mid - 1;
     }
     const size_t po = (size_t)p * n;
-    const int s = a.desc[4 * q], m = a.desc[4 * q + 1] + a.desc[4 * q + 2];
-    const int k = b.kcnt[po + q];
-    const int ntm = (m + 63) / 64;
-    const int64_t lt = t - a.desc[4 * q + 3];
-    const int ti = (int)(lt % ntm), tj = (int)(lt / ntm);
-    if (tj * 64 >= k || ti * 64 >= m) return;
-    GemmShape g{m, k, k, false, false, (int64_t)n, (int64_t)n, (int64_t)n, 0, 0, 0, 1, 0};
+    const int s = a.desc[4 * q], m1 = a.desc[4 * q + 1], m2 = a.desc[4 * q + 2], m = m1 + m2;
+    const int k = b.kcnt[po + q], k1 = b.kc1[po + q], k2 = b.kc2[po + q];
+    const int ntm = (m + 63) / 64, nt1 = (m1 + 63) / 64, nt2 = (m2 + 63) / 64;
+    int64_t lt = t - a.desc[4 * q + 3];
     const T* A = b.QK + po * n + (size_t)s * n;
     const T* B = b.UB + po * n + (size_t)s * n;
     T* C = b.QN + po * n + (size_t)s * n;
+    int rows, inner, ti, tj;
+    if (lt < (int64_t)nt1 * ntm) {
+        // first child's rows over columns [0, k1 + k2)
+        ti = (int)(lt % nt1);
+        tj = (int)(lt / nt1);
+        rows = m1;
+        inner = k1 + k2;
+    } else {
+        // second child's rows over columns [k1, k)
+        lt -= (int64_t)nt1 * ntm;
+        if (nt2 == 0) return;
+        ti = (int)(lt % nt2);
+        tj = (int)(lt / nt2);
+        rows = m2;
+        inner = k - k1;
+        A += m1 + (size_t)k1 * n;
+        B += k1;
+        C += m1;
+    }
+    if (tj * 64 >= k || ti * 64 >= rows) return;
+    GemmShape g{rows, k, inner, false, false, (int64_t)n, (int64_t)n, (int64_t)n, 0, 0, 0, 1, 0};
     gemm_tile<T, GemmBK<T>::value>(g, ti * 64, tj * 64, T(1), A, B, T(0), C);
 }
 
@@ -1481,7 +1535,8 @@ static int eig_layout(EigPlan<T>* p, char* base)
     take(p->UB, B * nn * sizeof(T));
     for (T** v : {&p->dsort, &p->zsort, &p->dK, &p->zK, &p->taus, &p->zh, &p->EV, &p->rotc, &p->rots})
         take(*v, B * n * sizeof(T));
-    for (int** v : {&p->perm, &p->lst, &p->lstD, &p->org, &p->rank, &p->rotp, &p->rotj, &p->kcnt, &p->rcnt})
+    for (int** v : {&p->perm, &p->lst, &p->lstD, &p->org, &p->rank, &p->rotp, &p->rotj, &p->kcnt, &p->rcnt,
+                    &p->kpos, &p->kc1, &p->kc2})
         take(*v, B * n * sizeof(int));
     take(p->Tf, B * 2 * nbb * nbb * sizeof(T));
     take(p->bar, (B * G + 1) * BAR_STRIDE * sizeof(unsigned));
@@ -1547,8 +1602,9 @@ static int eig_plan_host(int n, int batch, EigPlan<T>* p, std::vector<std::vecto
         for (auto& nd : lv) {
             int m = nd.m1 + nd.m2;
             desc.insert(desc.end(), {nd.s, nd.m1, nd.m2, (int)off});
+            // the merge product's tiles: rows of each child's part times the columns
             int64_t tm = (m + 63) / 64;
-            off += tm * tm;
+            off += ((nd.m1 + 63) / 64 + (nd.m2 + 63) / 64) * tm;
             maxm = std::max(maxm, m);
         }
         GMC_CHECK(off < (1LL << 31), "eig_plan: tile count overflow");
@@ -1786,7 +1842,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
 
     // ---------------- divide & conquer
     DcBufs<T> b{p.e, w, p.Q, p.QK, p.QN, p.UB, p.dsort, p.zsort, p.dK, p.zK, p.taus, p.zh, p.EV, p.rotc, p.rots,
-                p.perm, p.lst, p.lstD, p.org, p.rank, p.rotp, p.rotj, p.kcnt, p.rcnt};
+                p.perm, p.lst, p.lstD, p.org, p.rank, p.rotp, p.rotj, p.kcnt, p.rcnt, p.kpos, p.kc1, p.kc2};
     hipLaunchKernelGGL(dc_init_kernel<T>, dim3(grid1(nn), B), dim3(256), 0, s, p.d, p.e, p.d_split, w, p.Q, n);
     GMC_HIP(hipGetLastError());
     for (auto& L : p.levels) {

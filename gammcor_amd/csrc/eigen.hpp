@@ -42,6 +42,7 @@ struct EigPlan {
     T *EV = nullptr, *rotc = nullptr, *rots = nullptr, *Tf = nullptr;
     int *perm = nullptr, *lst = nullptr, *lstD = nullptr, *org = nullptr, *rank = nullptr;
     int *rotp = nullptr, *rotj = nullptr, *kcnt = nullptr, *rcnt = nullptr;
+    int *kpos = nullptr, *kc1 = nullptr, *kc2 = nullptr;   // merge GEMM column grouping
     unsigned* bar = nullptr;
     int* err = nullptr;
     T* fro = nullptr;   // deterministic mode: ||A_b||_F of each problem
