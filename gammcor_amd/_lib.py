@@ -60,6 +60,8 @@ SIGNATURES = {
     "gmc_set_deterministic": [_c_int],
     "gmc_gemm_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
                      _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
+    "gmc_gemm_lower_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
+                           _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
 }
 
 _lib = None

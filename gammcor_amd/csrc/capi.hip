@@ -432,6 +432,21 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
     return 0;
 }
 
+int gmc_gemm_lower_dev(int prec, int ta, int tb, int n, int k, double alpha, const void* A, int64_t lda,
+                       int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
+                       int64_t sc, int batch, void* stream)
+{
+    GMC_TRY(ensure_init());
+    GemmShape g{n, n, k, ta != 0, tb != 0, lda, ldb, ldc, sa, sb, sc, batch, 1};
+    hipStream_t s = dev_stream(stream);
+    if (prec == GMC_FP64)
+        GMC_TRY(gemm<double>(s, g, alpha, (const double*)A, (const double*)B, beta, (double*)C));
+    else
+        GMC_TRY(gemm<float>(s, g, (float)alpha, (const float*)A, (const float*)B, (float)beta, (float*)C));
+    GMC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
 int gmc_syevd_dev(int prec, int n, int batch, void* A, void* w, void* stream)
 {
     GMC_TRY(ensure_init());

@@ -155,6 +155,14 @@ int gmc_gemm_dev(int prec, int ta, int tb, int m, int n, int k, double alpha, co
                  int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
                  int64_t sc, int batch, void* stream);
 
+/* The same for a square n x n C of which only the lower triangle (i >= j)
+ * is formed and written; the strict upper triangle is left as it is (the
+ * eigensolver's trailing updates and the Cholesky's take this form).  No
+ * counterpart in the reference (DSYR2K/DSYRK in MKL's DSYTRD/DPOTRF). */
+int gmc_gemm_lower_dev(int prec, int ta, int tb, int n, int k, double alpha, const void* A, int64_t lda,
+                       int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
+                       int64_t sc, int batch, void* stream);
+
 /* ACECORR (acfd.f:2-297, alpha loop 161-201) for the CAS case with the
  * in-core integrals: ECorr = sum_i w_i W(alpha_i), every alpha point running
  * AB_CAS (interpa.f:3659) -> ERPASYMM1 (interpa.f:1652) -> ACEneERPA
