@@ -286,7 +286,8 @@ def main():
     PEAK = {"hbm": 8.0e12, "mfma_f64": 78.6e12, "mfma_f32": 157.3e12, "mfma_f16": 2.5e15}
     KERNELS = {"sytrd_panel_f32": ("hbm", PEAK["hbm"]), "sytrd_panel_f64": ("hbm", PEAK["hbm"]),
                "gemm_f64": ("mfma", PEAK["mfma_f64"]), "gemm_f32": ("mfma", PEAK["mfma_f32"]),
-               "ab_cas_main": ("hbm", PEAK["hbm"]), "blas_gemm_f16": ("mfma", PEAK["mfma_f16"])}
+               "ab_cas_main": ("hbm", PEAK["hbm"]), "ab_cas_tables": ("hbm", PEAK["hbm"]),
+               "blas_gemm_f16": ("mfma", PEAK["mfma_f16"])}
     PHASES = [f"eig_{ph}_{t}" for ph in ("tridiag", "dc", "backtr") for t in ("f32", "f64")] + ["twono1"]
     for ab in ("nn", "nt", "tn", "tt"):     # rocBLAS products, one kernel per transpose pair
         KERNELS[f"blas_gemm_f64_{ab}"] = ("mfma", PEAK["mfma_f64"])

@@ -172,7 +172,62 @@ __global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hn
 
 struct AbAux {
     const double *h0, *h1, *ai0, *ai1, *aio0, *aio1, *w0, *w1, *c;
+    // The active-space sums of T1..T6 contracted ahead (ab_gather_kernel,
+    // ab_gam_kernel and four MFMA products), N^2 x nA^2 each, column-major:
+    //   x12[(s + N q) + N^2 (p' + nA r')] = sum_tu (sq|tu) G(pu,rt) + (su|tq) G(pu,tr)
+    //   x34[(p + N r) + N^2 (s' + nA q')] = sum_tu (ut|pr) G(st,qu) + (ur|pt) G(st,uq)
+    //   x5 [(p + N s) + N^2 (q' + nA r')] = sum_tu (pt|su) G(tu,qr)
+    //   x6 [(q + N r) + N^2 (s' + nA p')] = sum_tu (tq|ur) G(sp,ut)
+    // (x' = x - nI for an active index; G the active 2-RDM); null without
+    // active orbitals
+    const double *x12, *x34, *x5, *x6;
 };
+
+// The integral blocks the contractions read, one thread per ((a, b), (t, u)),
+// a, b over all orbitals, t, u active; each N^2 x nA^2 block column-major
+// (row a + N b, column t' + nA u'), with the argument order of the loop
+// terms they replace:
+//   iab = [tw(a,b,t,u) | tw(a,u,t,b)], icd = [tw(u,t,a,b) | tw(u,b,a,t)],
+//   ie = tw(a,t,b,u), if_ = tw(t,a,u,b).
+__global__ void ab_gather_kernel(DevSys s, double* iab, double* icd, double* ie, double* if_)
+{
+    const int N = s.N, nA = s.nA, nI = s.nI, na2 = nA * nA;
+    const int64_t NN = (int64_t)N * N;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= NN * na2) return;
+    const int64_t row = idx % NN;
+    const int col = (int)(idx / NN);
+    const int a = (int)(row % N), b = (int)(row / N);
+    const int t = nI + col % nA, u = nI + col / nA;
+    iab[row + NN * col] = tw(s, a, b, t, u);
+    iab[row + NN * (na2 + col)] = tw(s, a, u, t, b);
+    icd[row + NN * col] = tw(s, u, t, a, b);
+    icd[row + NN * (na2 + col)] = tw(s, u, b, a, t);
+    ie[row + NN * col] = tw(s, a, t, b, u);
+    if_[row + NN * col] = tw(s, t, a, u, b);
+}
+
+// The 2-RDM arranged for the four products: row c = t' + nA u' (+ nA^2 for
+// the second term), column d = x' + nA y'.
+//   gab[c][d = p' + nA r'] = G(p,u,r,t) | G(p,u,t,r)
+//   gcd[c][d = s' + nA q'] = G(s,t,q,u) | G(s,t,u,q)
+//   ge [c][d = q' + nA r'] = G(t,u,q,r)
+//   gf [c][d = s' + nA p'] = G(s,p,u,t)
+__global__ void ab_gam_kernel(const double* gam, int nA, double* gab, double* gcd, double* ge, double* gf)
+{
+    const int na2 = nA * nA;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= na2 * na2) return;
+    const int c = idx % na2, d = idx / na2;
+    const int t = c % nA, u = c / nA, x = d % nA, y = d / nA;
+    auto G = [&](int i, int j, int k, int l) { return gam[((i * nA + j) * nA + k) * nA + l]; };
+    gab[c + 2 * na2 * d] = G(x, u, y, t);
+    gab[na2 + c + 2 * na2 * d] = G(x, u, t, y);
+    gcd[c + 2 * na2 * d] = G(x, t, y, u);
+    gcd[na2 + c + 2 * na2 * d] = G(x, t, u, y);
+    ge[c + na2 * d] = G(t, u, x, y);
+    gf[c + na2 * d] = G(x, y, u, t);
+}
 
 // A_{rs,pq} of interpa.f:3905-4054 at alpha = 0 (e0) and alpha = 1 (e1).
 __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, int ir, int is, double& e0,
@@ -194,6 +249,11 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
     const double f0pqrs = same4(g, ip, iq, ir, is) ? 1.0 : 0.0;
     const double xpqrs = 2.0 * tw(s, ip, iq, ir, is) - tw(s, ip, ir, iq, is);
     const int nI = s.nI, nA = s.nA;
+    const int64_t NN = (int64_t)N * N;
+    // an active orbital: its 2-RDM entries are the active 2-RDM's, and the
+    // term's sum over t, u is a table entry (an inactive or virtual orbital
+    // that reaches a loop, e.g. with an occupation just below 1, sums it here)
+    auto act = [&](int o) { return o >= nI && o < nI + nA; };
     // T1+T2
     if (occ[ip] * occ[ir] != 0.0) {
         if (occ[ip] == 1.0 || occ[ir] == 1.0) {
@@ -206,12 +266,16 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             }
         } else {
             double sum = 0.0;
-            for (int tt = 0; tt < nA; tt++)
-                for (int uu = 0; uu < nA; uu++) {
-                    const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s, is, iq, it, iu) * frdm2(s, ip, iu, ir, it) +
-                           tw(s, is, iu, it, iq) * frdm2(s, ip, iu, it, ir);
-                }
+            if (x.x12 && act(ip) && act(ir)) {
+                sum = x.x12[(is + (int64_t)N * iq) + NN * ((ip - nI) + nA * (ir - nI))];
+            } else {
+                for (int tt = 0; tt < nA; tt++)
+                    for (int uu = 0; uu < nA; uu++) {
+                        const int it = nI + tt, iu = nI + uu;
+                        sum += tw(s, is, iq, it, iu) * frdm2(s, ip, iu, ir, it) +
+                               tw(s, is, iu, it, iq) * frdm2(s, ip, iu, it, ir);
+                    }
+            }
             if (same4(g, is, iq, nI, nI)) a0 += sum;
             a1 += sum;
             if (ip == ir) {
@@ -232,12 +296,16 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             }
         } else {
             double sum = 0.0;
-            for (int tt = 0; tt < nA; tt++)
-                for (int uu = 0; uu < nA; uu++) {
-                    const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s, iu, it, ip, ir) * frdm2(s, is, it, iq, iu) +
-                           tw(s, iu, ir, ip, it) * frdm2(s, is, it, iu, iq);
-                }
+            if (x.x34 && act(iq) && act(is)) {
+                sum = x.x34[(ip + (int64_t)N * ir) + NN * ((is - nI) + nA * (iq - nI))];
+            } else {
+                for (int tt = 0; tt < nA; tt++)
+                    for (int uu = 0; uu < nA; uu++) {
+                        const int it = nI + tt, iu = nI + uu;
+                        sum += tw(s, iu, it, ip, ir) * frdm2(s, is, it, iq, iu) +
+                               tw(s, iu, ir, ip, it) * frdm2(s, is, it, iu, iq);
+                    }
+            }
             if (same4(g, nI, nI, ip, ir)) a0 += sum;
             a1 += sum;
             if (iq == is) {
@@ -254,11 +322,15 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             a1 -= v;
         } else {
             double sum = 0.0;
-            for (int tt = 0; tt < nA; tt++)
-                for (int uu = 0; uu < nA; uu++) {
-                    const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s, ip, it, is, iu) * frdm2(s, it, iu, iq, ir);
-                }
+            if (x.x5 && act(iq) && act(ir)) {
+                sum = x.x5[(ip + (int64_t)N * is) + NN * ((iq - nI) + nA * (ir - nI))];
+            } else {
+                for (int tt = 0; tt < nA; tt++)
+                    for (int uu = 0; uu < nA; uu++) {
+                        const int it = nI + tt, iu = nI + uu;
+                        sum += tw(s, ip, it, is, iu) * frdm2(s, it, iu, iq, ir);
+                    }
+            }
             if (same4(g, ip, nI, is, nI)) a0 -= sum;
             a1 -= sum;
         }
@@ -271,11 +343,15 @@ __device__ inline void arspq(const DevSys& s, const AbAux& x, int ip, int iq, in
             a1 -= v;
         } else {
             double sum = 0.0;
-            for (int tt = 0; tt < nA; tt++)
-                for (int uu = 0; uu < nA; uu++) {
-                    const int it = nI + tt, iu = nI + uu;
-                    sum += tw(s, it, iq, iu, ir) * frdm2(s, is, ip, iu, it);
-                }
+            if (x.x6 && act(ip) && act(is)) {
+                sum = x.x6[(iq + (int64_t)N * ir) + NN * ((is - nI) + nA * (ip - nI))];
+            } else {
+                for (int tt = 0; tt < nA; tt++)
+                    for (int uu = 0; uu < nA; uu++) {
+                        const int it = nI + tt, iu = nI + uu;
+                        sum += tw(s, it, iq, iu, ir) * frdm2(s, is, ip, iu, it);
+                    }
+            }
             if (same4(g, nI, iq, nI, ir)) a0 -= sum;
             a1 -= sum;
         }
@@ -345,8 +421,9 @@ static DevSys devsys(const AcSystem& a)
 size_t ab_scratch_bytes(const AcSystem& s)
 {
     const size_t N = s.N, T = N * (N + 1) / 2, nO = s.nI + s.nA;
+    const size_t na2 = (size_t)s.nA * s.nA, xs = N * N * na2 * 8;
     return carve_bytes({N * N * 8, N * N * 8, N * N * 8, T * 8, T * 8, T * 8, T * 8, T * 8, T * 8, N * nO * 8,
-                        N * nO * 8, 8});
+                        N * nO * 8, 8, 2 * xs, 2 * xs, xs, xs, xs, xs, xs, xs, 6 * na2 * na2 * 8});
 }
 
 int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
@@ -367,6 +444,20 @@ int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
     double* w0 = ar.take<double>((size_t)N * nO);
     double* w1 = ar.take<double>((size_t)N * nO);
     double* d_e = ar.take<double>(1);
+    const int nA = a.nA, na2 = nA * nA;
+    const size_t xs = (size_t)N * N * na2;
+    double* iab = ar.take<double>(2 * xs);
+    double* icd = ar.take<double>(2 * xs);
+    double* ie = ar.take<double>(xs);
+    double* if_ = ar.take<double>(xs);
+    double* x12 = ar.take<double>(xs);
+    double* x34 = ar.take<double>(xs);
+    double* x5 = ar.take<double>(xs);
+    double* x6 = ar.take<double>(xs);
+    double* gab = ar.take<double>(6 * (size_t)na2 * na2);
+    double* gcd = gab + 2 * (size_t)na2 * na2;
+    double* ge = gcd + 2 * (size_t)na2 * na2;
+    double* gf = ge + (size_t)na2 * na2;
     GMC_CHECK(ar.fits(), "AB_CAS: scratch layout overflow");
     DevSys s = devsys(a);
 
@@ -387,7 +478,32 @@ int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
         hipLaunchKernelGGL(ab_wmat_kernel, dim3((unsigned)(((int64_t)N * nO * 64 + 255) / 256)), dim3(256), 0, st, s,
                            w0, w1);
     GMC_HIP(hipGetLastError());
-    AbAux x{h0, h1, ai0, ai1, aio0, aio1, w0, w1, out.cicoef};
+    AbAux x{h0, h1, ai0, ai1, aio0, aio1, w0, w1, out.cicoef, nullptr, nullptr, nullptr, nullptr};
+    if (na2 > 0) {
+        // the active-space sums as products over (t, u): the integral blocks
+        // and the 2-RDM arrangements, then X = I G on the FP64 MFMA tile
+        // (algorithmic bytes: the ten N^2 x nA^2 tables written once and the
+        // six integral blocks read once by the products)
+        const int64_t NN = (int64_t)N * N;
+        ProfScope ps(st, "ab_cas_tables", 16.0 * 8.0 * (double)xs);
+        GMC_CHECK(NN <= INT32_MAX && NN * 2 * na2 < ((int64_t)1 << 40), "AB_CAS: N=%d, nA=%d too large", N, nA);
+        hipLaunchKernelGGL(ab_gather_kernel, dim3((unsigned)((NN * na2 + 255) / 256)), dim3(256), 0, st, s, iab, icd,
+                           ie, if_);
+        hipLaunchKernelGGL(ab_gam_kernel, dim3((na2 * na2 + 255) / 256), dim3(256), 0, st, a.gam, nA, gab, gcd, ge,
+                           gf);
+        GMC_HIP(hipGetLastError());
+        const int m = (int)NN;
+        GemmShape g2{m, na2, 2 * na2, false, false, NN, 2 * na2, NN, 0, 0, 0, 1, 0};
+        GMC_TRY(gemm<double>(st, g2, 1.0, iab, gab, 0.0, x12));
+        GMC_TRY(gemm<double>(st, g2, 1.0, icd, gcd, 0.0, x34));
+        GemmShape g1{m, na2, na2, false, false, NN, na2, NN, 0, 0, 0, 1, 0};
+        GMC_TRY(gemm<double>(st, g1, 1.0, ie, ge, 0.0, x5));
+        GMC_TRY(gemm<double>(st, g1, 1.0, if_, gf, 0.0, x6));
+        x.x12 = x12;
+        x.x34 = x34;
+        x.x5 = x5;
+        x.x6 = x6;
+    }
     GMC_CHECK(a.ndimx <= 65535, "AB_CAS: NDimX=%d exceeds the grid y limit", a.ndimx);
     {
         // algorithmic bytes: the four ndimx^2 output matrices written once
