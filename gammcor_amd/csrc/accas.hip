@@ -151,15 +151,20 @@ __global__ __launch_bounds__(256) void ab_wmat_kernel(DevSys s, double* w0, doub
     }
 }
 
-// ECASSCF (interpa.f:3755-3771) with HNO before the alpha modification.
-__global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hno, double* out)
+// ECASSCF (interpa.f:3755-3771) with HNO before the alpha modification:
+// ECAS_BLOCKS workgroups each sum a strided share of the one- and
+// two-electron terms into part[blockIdx.x]; ab_ecas_sum_kernel adds the
+// partials in block order (the same bits on every run).
+constexpr int ECAS_BLOCKS = 128;
+__global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hno, double* part)
 {
     __shared__ double red[4];
+    const int64_t stride = (int64_t)ECAS_BLOCKS * 256, t0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double e = 0.0;
-    for (int i = threadIdx.x; i < s.N; i += 256) e += 2.0 * s.occ[i] * hno[i + (size_t)i * s.N];
+    for (int64_t i = t0; i < s.N; i += stride) e += 2.0 * s.occ[i] * hno[i + (size_t)i * s.N];
     const int nO = s.nO;
     const int64_t tot = (int64_t)nO * nO * nO * nO;
-    for (int64_t x = threadIdx.x; x < tot; x += 256) {
+    for (int64_t x = t0; x < tot; x += stride) {
         const int p = (int)(x / ((int64_t)nO * nO * nO)), q = (int)((x / (nO * nO)) % nO), r = (int)((x / nO) % nO),
                   ss = (int)(x % nO);
         e += frdm2(s, p, q, r, ss) * tw(s, p, r, q, ss);
@@ -167,7 +172,14 @@ __global__ __launch_bounds__(256) void ab_ecas_kernel(DevSys s, const double* hn
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
-    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void ab_ecas_sum_kernel(const double* part, double* out)
+{
+    if (threadIdx.x != 0) return;
+    double e = 0.0;
+    for (int b = 0; b < ECAS_BLOCKS; b++) e += part[b];
+    *out = e;
 }
 
 struct AbAux {
@@ -423,7 +435,8 @@ size_t ab_scratch_bytes(const AcSystem& s)
     const size_t N = s.N, T = N * (N + 1) / 2, nO = s.nI + s.nA;
     const size_t na2 = (size_t)s.nA * s.nA, xs = N * N * na2 * 8;
     return carve_bytes({N * N * 8, N * N * 8, N * N * 8, T * 8, T * 8, T * 8, T * 8, T * 8, T * 8, N * nO * 8,
-                        N * nO * 8, 8, 2 * xs, 2 * xs, xs, xs, xs, xs, xs, xs, 6 * na2 * na2 * 8});
+                        N * nO * 8, 8, ECAS_BLOCKS * 8, 2 * xs, 2 * xs, xs, xs, xs, xs, xs, xs,
+                        6 * na2 * na2 * 8});
 }
 
 int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
@@ -444,6 +457,7 @@ int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
     double* w0 = ar.take<double>((size_t)N * nO);
     double* w1 = ar.take<double>((size_t)N * nO);
     double* d_e = ar.take<double>(1);
+    double* d_epart = ar.take<double>(ECAS_BLOCKS);
     const int nA = a.nA, na2 = nA * nA;
     const size_t xs = (size_t)N * N * na2;
     double* iab = ar.take<double>(2 * xs);
@@ -470,7 +484,8 @@ int assemble_ab01(hipStream_t st, const AcSystem& a, void* scratch, AbPair& out)
         GemmShape g2{N, N, N, false, true, N, N, N, 0, 0, 0, 1, 0};
         GMC_TRY(gemm<double>(st, g2, 1.0, UH, a.ure, 0.0, hno));
     }
-    hipLaunchKernelGGL(ab_ecas_kernel, dim3(1), dim3(256), 0, st, s, hno, d_e);
+    hipLaunchKernelGGL(ab_ecas_kernel, dim3(ECAS_BLOCKS), dim3(256), 0, st, s, hno, d_epart);
+    hipLaunchKernelGGL(ab_ecas_sum_kernel, dim3(1), dim3(64), 0, st, d_epart, d_e);
     hipLaunchKernelGGL(cicoef_kernel, dim3((N + 255) / 256), dim3(256), 0, st, a.occ, out.cicoef, N);
     hipLaunchKernelGGL(ab_aux_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, s, hno, h0, h1, ai0, ai1,
                        aio0, aio1);
