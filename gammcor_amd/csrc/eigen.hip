@@ -832,15 +832,20 @@ __global__ void clear_upper_kernel(T* A, int n)
 // the Gram matrix Gm = V^T V (ld nbb) and tau: column i of T is
 // -tau_i T(0:i, 0:i) G(0:i, i), diagonal tau_i.  T, the upper triangle of G
 // and tau live in LDS (dynamic, kb (kb + 1) + kb elements): upper triangles
-// packed by columns, column c at c (c + 1) / 2.
+// packed by columns, column c at c (c + 1) / 2.  Workgroup (p, q) forms the
+// factor of problem p's block k = k0 + q nbb (kb = min(nbb, nu - 1 - k)):
+// Gram matrix and factor at (q B + p) nbb^2, so one launch can form every
+// block's factor of a back-transformation.
 constexpr int BT_MAX = 128;   // largest back-transformation block
 template <class T>
-__global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int k, int kb, int nbb)
+__global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int nu, int k0, int nbb)
 {
     extern __shared__ char smem[];
-    const int p = blockIdx.x;
-    const T* G = Gm + (size_t)p * nbb * nbb;
-    T* Tm = Tf + (size_t)p * nbb * nbb;
+    const int p = blockIdx.x, q = blockIdx.y;
+    const int k = k0 + q * nbb, kb = min(nbb, nu - 1 - k);
+    const size_t at = ((size_t)q * gridDim.x + p) * nbb * nbb;
+    const T* G = Gm + at;
+    T* Tm = Tf + at;
     const int tri = kb * (kb + 1) / 2;
     T* t = reinterpret_cast<T*>(smem);
     T* gs = t + tri;
@@ -1880,16 +1885,41 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
         const int nu = p.n;
         const int64_t nun = (int64_t)nu * nu;
         const int last = ((nu - 2) / nbb) * nbb;
+        // Every block's T factor depends on its reflectors alone: when the
+        // Gram matrices and factors of all blocks fit in the divide &
+        // conquer's UB (free from here on), they are formed before the loop,
+        // the factors in one launch over (problem, block), instead of one
+        // sequential 16-workgroup launch per block
+        const int nblk = last / nbb + 1;
+        const size_t blk = (size_t)B * nbb * nbb;
+        const bool hoist = 2 * (size_t)nblk * nbb * nbb <= nn;
+        T* Gall = p.UB;
+        T* Tall = p.UB + (size_t)nblk * blk;
+        if (hoist) {
+            for (int k = 0; k <= last; k += nbb) {
+                const int kb = std::min(nbb, nu - 1 - k);
+                const T* V = A + (k + 1) + (size_t)k * nu;
+                GemmShape g1{kb, kb, nu - k - 1, true, false, (int64_t)nu, (int64_t)nu, (int64_t)nbb, nun, nun,
+                             (int64_t)nbb * nbb, B, 0};
+                GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gall + (size_t)(k / nbb) * blk));
+            }
+            hipLaunchKernelGGL(larft_kernel<T>, dim3(B, nblk), dim3(256), lds_t, s, Gall, p.tau, Tall, n, nu, 0, nbb);
+            GMC_HIP(hipGetLastError());
+        }
         for (int k = last; k >= 0; k -= nbb) {
             const int kb = std::min(nbb, nu - 1 - k);
             const int m = nu - k - 1;
             const T* V = A + (k + 1) + (size_t)k * nu;
             T* Zs = p.Q + (k + 1);
-            GemmShape g1{kb, kb, m, true, false, (int64_t)nu, (int64_t)nu, (int64_t)nbb, nun, nun,
-                         (int64_t)nbb * nbb, B, 0};
-            GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gm));
-            hipLaunchKernelGGL(larft_kernel<T>, dim3(B), dim3(256), lds_t, s, Gm, p.tau, Tm, n, k, kb, nbb);
-            GMC_HIP(hipGetLastError());
+            if (hoist) {
+                Tm = Tall + (size_t)(k / nbb) * blk;
+            } else {
+                GemmShape g1{kb, kb, m, true, false, (int64_t)nu, (int64_t)nu, (int64_t)nbb, nun, nun,
+                             (int64_t)nbb * nbb, B, 0};
+                GMC_TRY(gemm<T>(s, g1, T(1), V, V, T(0), Gm));
+                hipLaunchKernelGGL(larft_kernel<T>, dim3(B, 1), dim3(256), lds_t, s, Gm, p.tau, Tm, n, nu, k, nbb);
+                GMC_HIP(hipGetLastError());
+            }
             GemmShape g2{kb, nu, m, true, false, (int64_t)nu, (int64_t)n, (int64_t)nbb, nun, (int64_t)nn,
                          (int64_t)nn, B, 0};
             // own_gemm: the library's MFMA tile (no library workspace)
