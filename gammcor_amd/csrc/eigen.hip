@@ -830,62 +830,47 @@ __global__ void clear_upper_kernel(T* A, int n)
 // ====================================================================== ormtr
 // T factor (upper triangular, ld nbb) of H(k)...H(k+kb-1) = I - V T V^T from
 // the Gram matrix Gm = V^T V (ld nbb) and tau: column i of T is
-// -tau_i T(0:i, 0:i) G(0:i, i), diagonal tau_i.  T, the upper triangle of G
-// and tau live in LDS (dynamic, kb (kb + 1) + kb elements): upper triangles
-// packed by columns, column c at c (c + 1) / 2.  Workgroup (p, q) forms the
-// factor of problem p's block k = k0 + q nbb (kb = min(nbb, nu - 1 - k)):
-// Gram matrix and factor at (q B + p) nbb^2, so one launch can form every
-// block's factor of a back-transformation.
-constexpr int BT_MAX = 128;   // largest back-transformation block
+// -tau_i T(0:i, 0:i) G(0:i, i), diagonal tau_i.  T lives in LDS as its
+// upper triangle packed by columns (column c at c (c + 1) / 2); column i of
+// G is read into LDS at step i, with tau (dynamic LDS: kb (kb + 1) / 2 + 2 kb
+// elements).  Workgroup (p, q) forms the factor of problem p's block
+// k = k0 + q nbb (kb = min(nbb, nu - 1 - k)): Gram matrix and factor at
+// (q B + p) nbb^2, so one launch can form every block's factor of a
+// back-transformation.
+template <class T>
+constexpr int bt_max() { return sizeof(T) == 4 ? 256 : 128; }   // largest back-transformation block
 template <class T>
 __global__ __launch_bounds__(256) void larft_kernel(const T* Gm, const T* tau, T* Tf, int n, int nu, int k0, int nbb)
 {
     extern __shared__ char smem[];
     const int p = blockIdx.x, q = blockIdx.y;
     const int k = k0 + q * nbb, kb = min(nbb, nu - 1 - k);
-    const size_t at = ((size_t)q * gridDim.x + p) * nbb * nbb;
-    const T* G = Gm + at;
-    T* Tm = Tf + at;
+    const size_t at0 = ((size_t)q * gridDim.x + p) * nbb * nbb;
+    const T* G = Gm + at0;
+    T* Tm = Tf + at0;
     const int tri = kb * (kb + 1) / 2;
     T* t = reinterpret_cast<T*>(smem);
-    T* gs = t + tri;
-    T* ts = gs + tri;
-    // the upper triangle of G, eight independent loads in flight per thread
-    const int sq = kb * kb;
-    for (int b0 = 0; b0 < sq; b0 += 8 * 256) {
-        T gv[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int idx = b0 + e * 256 + threadIdx.x, r = idx % kb, c = idx / kb;
-            gv[e] = (idx < sq && r <= c) ? G[r + (size_t)c * nbb] : T(0);
-        }
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int idx = b0 + e * 256 + threadIdx.x, r = idx % kb, c = idx / kb;
-            if (idx < sq && r <= c) {
-                gs[c * (c + 1) / 2 + r] = gv[e];
-                t[c * (c + 1) / 2 + r] = T(0);
-            }
-        }
-    }
+    T* gcol = t + tri;
+    T* ts = gcol + kb;
     for (int i = threadIdx.x; i < kb; i += 256) ts[i] = tau[(size_t)p * n + k + i];
-    __syncthreads();
     for (int i = 0; i < kb; i++) {
         const int ci = i * (i + 1) / 2;
+        for (int c = threadIdx.x; c < i; c += 256) gcol[c] = G[c + (size_t)i * nbb];
+        __syncthreads();
         if ((int)threadIdx.x < i) {
             const int c = threadIdx.x;
             // four independent terms per pass keep LDS loads in flight
             T acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
             int c2 = c, at = c * (c + 1) / 2 + c;   // at = position of t(c, c2)
             for (; c2 + 3 < i; c2 += 4) {
-                acc0 += t[at] * gs[ci + c2];
-                acc1 += t[at + c2 + 1] * gs[ci + c2 + 1];
-                acc2 += t[at + 2 * c2 + 3] * gs[ci + c2 + 2];
-                acc3 += t[at + 3 * c2 + 6] * gs[ci + c2 + 3];
+                acc0 += t[at] * gcol[c2];
+                acc1 += t[at + c2 + 1] * gcol[c2 + 1];
+                acc2 += t[at + 2 * c2 + 3] * gcol[c2 + 2];
+                acc3 += t[at + 3 * c2 + 6] * gcol[c2 + 3];
                 at += 4 * c2 + 10;
             }
             for (; c2 < i; c2++) {
-                acc0 += t[at] * gs[ci + c2];
+                acc0 += t[at] * gcol[c2];
                 at += c2 + 1;
             }
             t[ci + c] = -ts[i] * ((acc0 + acc1) + (acc2 + acc3));
@@ -1623,7 +1608,7 @@ static int eig_plan_host(int n, int batch, EigPlan<T>* p, std::vector<std::vecto
 
     // back-transformation block: the rank-nbb updates of Z read and write Z
     // once per block, so the block is as wide as the T factor in LDS allows
-    p->nbb = std::min(BT_MAX, n);
+    p->nbb = std::min(bt_max<T>(), n);
     GMC_TRY(eig_layout(p, nullptr));
     return 0;
 }
@@ -1875,7 +1860,7 @@ int syevd_batched(hipStream_t s, EigPlan<T>& p, T* A, T* w)
     T* Gm = p.Tf;
     T* Tm = p.Tf + (size_t)B * nbb * nbb;
     if (n > 1) {
-        const size_t lds_t = ((size_t)nbb * (nbb + 1) + nbb) * sizeof(T);
+        const size_t lds_t = ((size_t)nbb * (nbb + 1) / 2 + 2 * (size_t)nbb) * sizeof(T);
         if (!capturing(s))
             GMC_HIP(hipFuncSetAttribute((const void*)larft_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)lds_t));

@@ -29,7 +29,7 @@ struct EigPlan {
     // padded problem diag(A, c I) has c above the spectrum of A
     int np = 0;
     int nb = 32;            // tridiagonalisation panel width
-    int nbb = 128;          // back-transformation block width (<= 128)
+    int nbb = 128;          // back-transformation block width (<= 256 in FP32, 128 in FP64)
     int G = 1;              // workgroups cooperating on one problem
     std::vector<DcLevel> levels;
     unsigned char* d_split = nullptr;   // split flags (n)
