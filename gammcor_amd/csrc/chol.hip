@@ -7,7 +7,10 @@
 // each column a triangular product with the part already inverted); the
 // panel below becomes L21 = A21 L11^-T (a product with the small inverse)
 // and the trailing lower triangle takes A22 -= L21 L21^T, both on the
-// library's own MFMA tile (the trailing update as one lower-triangle launch).
+// library's own MFMA tile.  The update is split with look-ahead: the next
+// panel's columns first, then the next diagonal block is factored on the
+// caller's stream while the lower triangle past that panel is updated on a
+// second stream (one lower-triangle launch).
 #include "chol.hpp"
 #include "gemm.hpp"
 
@@ -126,17 +129,37 @@ int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info)
     double* Linv = static_cast<double*>(work);
     double* P = Linv + (size_t)B * CHOL_NB * CHOL_NB;
     const int64_t nn = (int64_t)n * n;
-    static bool attr = false;
-    if (!attr) {
+    // look-ahead: the trailing update past the next panel runs on a second
+    // stream (default priority) while the next diagonal block is factored;
+    // the stream and its two events are made once per device
+    constexpr int MAX_DEV = 16;
+    static bool ready[MAX_DEV] = {};
+    static hipStream_t s2s[MAX_DEV];
+    static hipEvent_t evn[MAX_DEV], evr[MAX_DEV];
+    int dev = 0;
+    GMC_HIP(hipGetDevice(&dev));
+    GMC_CHECK(dev >= 0 && dev < MAX_DEV, "potrf: device %d out of range", dev);
+    if (!ready[dev]) {
         GMC_HIP(hipFuncSetAttribute((const void*)potrf_diag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)DIAG_LDS));
-        attr = true;
+        GMC_HIP(hipStreamCreateWithFlags(&s2s[dev], hipStreamNonBlocking));
+        GMC_HIP(hipEventCreateWithFlags(&evn[dev], hipEventDisableTiming));
+        GMC_HIP(hipEventCreateWithFlags(&evr[dev], hipEventDisableTiming));
+        ready[dev] = true;
     }
+    hipStream_t s2 = s2s[dev];
+    hipEvent_t ev_next = evn[dev], ev_rest = evr[dev];
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    GMC_HIP(hipStreamIsCapturing(s, &cst));
+    const bool ahead = cst == hipStreamCaptureStatusNone;   // (a capture keeps to one stream)
+    bool rest_pending = false;
     hipLaunchKernelGGL(zero_info_kernel, dim3((B + 255) / 256), dim3(256), 0, s, info, B);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(B), dim3(256), DIAG_LDS, s, A, n, 0, std::min(CHOL_NB, n), Linv,
+                       info);
+    GMC_HIP(hipGetLastError());
     for (int k = 0; k < n; k += CHOL_NB) {
+        // the diagonal block at k is factored (and inverted into Linv)
         const int kb = std::min(CHOL_NB, n - k), m = n - k - kb;
-        hipLaunchKernelGGL(potrf_diag_kernel, dim3(B), dim3(256), DIAG_LDS, s, A, n, k, kb, Linv, info);
-        GMC_HIP(hipGetLastError());
         if (m == 0) break;
         const unsigned gp = (unsigned)std::min<int64_t>(4096, ((int64_t)m * kb + 255) / 256);
         hipLaunchKernelGGL(potrf_copy_panel_kernel, dim3(gp, B), dim3(256), 0, s, A, n, k, kb, P);
@@ -146,10 +169,32 @@ int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info)
         GemmShape gp21{m, kb, kb, false, true, n, CHOL_NB, n, (int64_t)n * CHOL_NB, (int64_t)CHOL_NB * CHOL_NB, nn, B,
                        0};
         GMC_TRY(gemm<double>(s, gp21, 1.0, P, Linv, 0.0, A21));
-        // A22 -= L21 L21^T (lower triangle, one launch)
-        GemmShape g22{m, m, kb, false, true, n, n, n, nn, nn, nn, B, 1};
-        GMC_TRY(gemm<double>(s, g22, -1.0, A21, A21, 1.0, A21 + (size_t)kb * n));
+        // A22 -= L21 L21^T: the next panel's columns first (all m rows, so
+        // the strict upper part of its diagonal block takes garbage that the
+        // final clear removes), then the lower triangle past them
+        const int kb2 = std::min(CHOL_NB, m), m2 = m - kb2;
+        if (rest_pending) GMC_HIP(hipStreamWaitEvent(s, ev_rest, 0));   // the columns the last rest updated
+        rest_pending = false;
+        GemmShape gnx{m, kb2, kb, false, true, n, n, n, nn, nn, nn, B, 0};
+        GMC_TRY(gemm<double>(s, gnx, -1.0, A21, A21, 1.0, A21 + (size_t)kb * n));
+        if (m2 > 0) {
+            double* A21r = A21 + kb2;
+            GemmShape g22{m2, m2, kb, false, true, n, n, n, nn, nn, nn, B, 1};
+            if (ahead) {
+                GMC_HIP(hipEventRecord(ev_next, s));
+                GMC_HIP(hipStreamWaitEvent(s2, ev_next, 0));
+                GMC_TRY(gemm<double>(s2, g22, -1.0, A21r, A21r, 1.0, A21r + (size_t)(kb + kb2) * n));
+                GMC_HIP(hipEventRecord(ev_rest, s2));
+                rest_pending = true;
+            } else {
+                GMC_TRY(gemm<double>(s, g22, -1.0, A21r, A21r, 1.0, A21r + (size_t)(kb + kb2) * n));
+            }
+        }
+        // the next diagonal block, while the rest of the update runs
+        hipLaunchKernelGGL(potrf_diag_kernel, dim3(B), dim3(256), DIAG_LDS, s, A, n, k + kb, kb2, Linv, info);
+        GMC_HIP(hipGetLastError());
     }
+    if (rest_pending) GMC_HIP(hipStreamWaitEvent(s, ev_rest, 0));
     const unsigned gz = (unsigned)std::min<int64_t>(65536, (nn + 255) / 256);
     hipLaunchKernelGGL(zero_strict_upper_kernel, dim3(gz, B), dim3(256), 0, s, A, n);
     GMC_HIP(hipGetLastError());
