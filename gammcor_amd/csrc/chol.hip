@@ -116,7 +116,25 @@ __global__ void zero_info_kernel(int* info, int B)
     if (b < B) info[b] = 0;
 }
 
+// The look-ahead's second stream and events, made on a device's first
+// factorisation and kept until potrf_release
+constexpr int CHOL_MAX_DEV = 16;
+bool ready[CHOL_MAX_DEV] = {};
+hipStream_t s2s[CHOL_MAX_DEV];
+hipEvent_t evn[CHOL_MAX_DEV], evr[CHOL_MAX_DEV];
+
 }  // namespace
+
+void potrf_release()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= CHOL_MAX_DEV || !ready[dev]) return;
+    (void)hipStreamSynchronize(s2s[dev]);
+    (void)hipEventDestroy(evn[dev]);
+    (void)hipEventDestroy(evr[dev]);
+    (void)hipStreamDestroy(s2s[dev]);
+    ready[dev] = false;
+}
 
 size_t potrf_work_bytes(int n, int B)
 {
@@ -132,13 +150,9 @@ int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info)
     // look-ahead: the trailing update past the next panel runs on a second
     // stream (default priority) while the next diagonal block is factored;
     // the stream and its two events are made once per device
-    constexpr int MAX_DEV = 16;
-    static bool ready[MAX_DEV] = {};
-    static hipStream_t s2s[MAX_DEV];
-    static hipEvent_t evn[MAX_DEV], evr[MAX_DEV];
     int dev = 0;
     GMC_HIP(hipGetDevice(&dev));
-    GMC_CHECK(dev >= 0 && dev < MAX_DEV, "potrf: device %d out of range", dev);
+    GMC_CHECK(dev >= 0 && dev < CHOL_MAX_DEV, "potrf: device %d out of range", dev);
     if (!ready[dev]) {
         GMC_HIP(hipFuncSetAttribute((const void*)potrf_diag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)DIAG_LDS));

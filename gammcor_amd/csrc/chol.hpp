@@ -16,4 +16,8 @@ size_t potrf_work_bytes(int n, int B);
 // such a problem is left unspecified.  work: potrf_work_bytes(n, B) bytes.
 int potrf_batched(hipStream_t s, int n, int B, double* A, void* work, int* info);
 
+// Releases the current device's look-ahead stream and events (made again by
+// the next factorisation).
+void potrf_release();
+
 }  // namespace gmc

@@ -6,6 +6,7 @@
 #include <tuple>
 #include <vector>
 
+#include "chol.hpp"
 #include "common.hpp"
 #include "eigen.hpp"
 #include "gemm.hpp"
@@ -67,6 +68,7 @@ struct State {
         eig32.clear();
         blas_release(blas);
         blas = nullptr;
+        potrf_release();
         for (Workspace* w : {&scratch, &resident, &acws, &erpaws, &sysio, &io, &fofo, &resolve, &batch})
             if (w->ptr) (void)hipFree(w->ptr);
         scratch = Workspace{};
