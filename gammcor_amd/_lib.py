@@ -62,6 +62,7 @@ SIGNATURES = {
                      _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
     "gmc_gemm_lower_dev": [_c_int, _c_int, _c_int, _c_int, _c_int, _c_double, _vp, _c_i64, _c_i64, _vp,
                            _c_i64, _c_i64, _c_double, _vp, _c_i64, _c_i64, _c_int, _vp],
+    "gmc_potrf_dev": [_c_int, _c_int, _vp, _vp, _vp],
 }
 
 _lib = None

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/gammcor_hip.h"
+#include "chol.hpp"
 #include "state.hpp"
 #include "acrun.hpp"
 #include "common.hpp"
@@ -443,6 +444,21 @@ int gmc_gemm_lower_dev(int prec, int ta, int tb, int n, int k, double alpha, con
         GMC_TRY(gemm<double>(s, g, alpha, (const double*)A, (const double*)B, beta, (double*)C));
     else
         GMC_TRY(gemm<float>(s, g, (float)alpha, (const float*)A, (const float*)B, (float)beta, (float*)C));
+    GMC_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int gmc_potrf_dev(int n, int batch, double* A, int* info, void* stream)
+{
+    GMC_TRY(ensure_init());
+    GMC_CHECK(n >= 1 && batch >= 1, "potrf: n=%d, batch=%d", n, batch);
+    State& st = state();
+    hipStream_t s = dev_stream(stream);
+    const size_t wb = potrf_work_bytes(n, batch), ib = align_up((size_t)batch * sizeof(int));
+    GMC_TRY(st.scratch.reserve(align_up(wb) + ib));
+    int* dinfo = reinterpret_cast<int*>(static_cast<char*>(st.scratch.ptr) + align_up(wb));
+    GMC_TRY(potrf_batched(s, n, batch, A, st.scratch.ptr, dinfo));
+    GMC_HIP(hipMemcpyAsync(info, dinfo, (size_t)batch * sizeof(int), hipMemcpyDeviceToHost, s));
     GMC_HIP(hipStreamSynchronize(s));
     return 0;
 }

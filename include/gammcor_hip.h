@@ -163,6 +163,15 @@ int gmc_gemm_lower_dev(int prec, int ta, int tb, int n, int k, double alpha, con
                        int64_t sa, const void* B, int64_t ldb, int64_t sb, double beta, void* C, int64_t ldc,
                        int64_t sc, int batch, void* stream);
 
+/* Building block exported for tests: batched FP64 Cholesky factorisation
+ * A_b = L_b L_b^T in place (batch column-major n x n matrices, stride n^2, the
+ * lower triangle read; L returned with its strict upper triangle zero), as
+ * the mixed path factors |A+|.  info (host, batch ints): 0, or the 1-based
+ * column whose pivot was not positive.  No counterpart in the reference (the
+ * reference takes |A+|^(1/2) from an eigendecomposition, interpa.f:1693-1716;
+ * DPOTRF is the LAPACK routine this replaces). */
+int gmc_potrf_dev(int n, int batch, double* A, int* info, void* stream);
+
 /* ACECORR (acfd.f:2-297, alpha loop 161-201) for the CAS case with the
  * in-core integrals: ECorr = sum_i w_i W(alpha_i), every alpha point running
  * AB_CAS (interpa.f:3659) -> ERPASYMM1 (interpa.f:1652) -> ACEneERPA
