@@ -41,21 +41,43 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
         for (int c = half; c < kb; c += 2) a[c][r] = r >= c ? Ab[(size_t)(k + c) * n + k + r] : 0.0;
     if (t == 0) bad = 0;
     __syncthreads();
-    for (int j = 0; j < kb; j++) {
-        const double p = a[j][j];   // the same value in every thread: the branch is uniform
-        if (!(p > 0.0 && p < INFINITY)) {
+    // two columns per step: the 2 x 2 pivot block from values every thread
+    // reads alike (so every branch is uniform), columns j and j+1 of L, then
+    // the rank-2 update of the block's remaining lower triangle
+    int j = 0;
+    for (; j + 1 < kb; j += 2) {
+        const double p0 = a[j][j];
+        if (!(p0 > 0.0 && p0 < INFINITY)) {
             if (t == 0) bad = k + j + 1;
             break;
         }
-        const double l = sqrt(p);
-        __syncthreads();
-        if (half == 0 && r >= j && r < kb) a[j][r] = r == j ? l : a[j][r] / l;
-        __syncthreads();
-        if (r > j && r < kb) {
-            const double lr = a[j][r];
-            for (int c = j + 1 + half; c <= r; c += 2) a[c][r] -= lr * a[j][c];
+        const double l0 = sqrt(p0), l10 = a[j][j + 1] / l0;
+        const double p1 = a[j + 1][j + 1] - l10 * l10;
+        if (!(p1 > 0.0 && p1 < INFINITY)) {
+            if (t == 0) bad = k + j + 2;
+            break;
         }
+        const double l1 = sqrt(p1);
+        const double x0 = r < kb ? a[j][r] : 0.0, x1 = r < kb ? a[j + 1][r] : 0.0;
+        const double L0 = r == j ? l0 : x0 / l0;
+        const double L1 = r == j + 1 ? l1 : (x1 - L0 * l10) / l1;
         __syncthreads();
+        if (half == 0 && r >= j && r < kb) a[j][r] = L0;
+        if (half == 1 && r >= j + 1 && r < kb) a[j + 1][r] = L1;
+        __syncthreads();
+        if (r > j + 1 && r < kb)
+            for (int c = j + 2 + half; c <= r; c += 2) a[c][r] -= L0 * a[j][c] + L1 * a[j + 1][c];
+        __syncthreads();
+    }
+    if (j == kb - 1 && !bad) {
+        // an odd block's last column
+        const double p = a[j][j];
+        if (!(p > 0.0 && p < INFINITY)) {
+            if (t == 0) bad = k + j + 1;
+        } else {
+            __syncthreads();
+            if (t == 0) a[j][j] = sqrt(p);
+        }
     }
     __syncthreads();
     if (bad) {
