@@ -19,7 +19,7 @@ namespace gmc {
 namespace {
 
 constexpr int LDSW = CHOL_NB + 1;   // padded column of the LDS block
-constexpr size_t DIAG_LDS = ((size_t)CHOL_NB * LDSW + 2 * CHOL_NB) * sizeof(double);
+constexpr size_t DIAG_LDS = ((size_t)CHOL_NB * LDSW + 4 * CHOL_NB) * sizeof(double);
 static_assert(CHOL_NB == 128, "potrf_diag_kernel maps 256 threads to 128 rows");
 
 // Factor of the kb x kb diagonal block at (k, k), written back, and its
@@ -87,21 +87,59 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int n, int k
     if (r < kb)
         for (int c = half; c < kb; c += 2) Ab[(size_t)(k + c) * n + k + r] = r >= c ? a[c][r] : 0.0;
     // in-place inverse, columns from the right: column j of L^-1 below the
-    // diagonal is -(1/l_jj) Linv[j+1:, j+1:] l[j+1:, j]; the two threads of
-    // a row split the sum by parity and combine through tmp2
+    // diagonal is -(1/l_jj) Linv[j+1:, j+1:] l[j+1:, j].  Two columns per
+    // step: one pass forms both sums over the columns already inverted,
+    //   S_j(r) = sum_{q>j} Linv(r,q) l(q,j),  S_i(r) = sum_{q>j} Linv(r,q) l(q,i),
+    // i = j - 1; then Linv(r,j) = -S_j(r)/l_jj and Linv(r,i) =
+    // -(Linv(r,j) l(j,i) + S_i(r))/l_ii.  The two threads of a row split the
+    // sums by parity and combine through tmp2/tmp4 (the lone column of an
+    // odd block takes the one-column form)
     double* tmp2 = tmp + CHOL_NB;
-    for (int j = kb - 1; j >= 0; j--) {
+    double* tmp3 = tmp2 + CHOL_NB;
+    double* tmp4 = tmp3 + CHOL_NB;
+    j = kb - 1;
+    for (; j >= 1; j -= 2) {
+        const int i = j - 1;
         __syncthreads();
-        const double djj = 1.0 / a[j][j];
+        const double djj = 1.0 / a[j][j], dii = 1.0 / a[i][i], lji = a[i][j];
         if (half == 0 && r > j && r < kb) tmp[r] = a[j][r];
+        if (half == 1 && r > j && r < kb) tmp3[r] = a[i][r];
+        __syncthreads();
+        double sj = 0.0, si = 0.0;
+        if (r > j && r < kb)
+            for (int q = j + 1 + half; q <= r; q += 2) {
+                const double v = a[q][r];
+                sj += v * tmp[q];
+                si += v * tmp3[q];
+            }
+        if (half == 1) {
+            tmp2[r] = sj;
+            tmp4[r] = si;
+        }
+        __syncthreads();
+        if (half == 0 && r > j && r < kb) {
+            const double lrj = -djj * (sj + tmp2[r]);
+            a[j][r] = lrj;
+            a[i][r] = -dii * (lrj * lji + si + tmp4[r]);
+        }
+        if (t == 0) {
+            a[j][j] = djj;
+            a[i][j] = -dii * djj * lji;
+            a[i][i] = dii;
+        }
+    }
+    if (j == 0) {
+        __syncthreads();
+        const double d00 = 1.0 / a[0][0];
+        if (half == 0 && r > 0 && r < kb) tmp[r] = a[0][r];
         __syncthreads();
         double s = 0.0;
-        if (r > j && r < kb)
-            for (int q = j + 1 + half; q <= r; q += 2) s += a[q][r] * tmp[q];
+        if (r > 0 && r < kb)
+            for (int q = 1 + half; q <= r; q += 2) s += a[q][r] * tmp[q];
         if (half == 1) tmp2[r] = s;
         __syncthreads();
-        if (half == 0 && r > j && r < kb) a[j][r] = -djj * (s + tmp2[r]);
-        if (t == 0) a[j][j] = djj;
+        if (half == 0 && r > 0 && r < kb) a[0][r] = -d00 * (s + tmp2[r]);
+        if (t == 0) a[0][0] = d00;
     }
     __syncthreads();
     double* Lb = Linv + (size_t)b * CHOL_NB * CHOL_NB;
